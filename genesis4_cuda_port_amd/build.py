@@ -1,0 +1,94 @@
+"""Build the gfx950 library libg4hip.so in-tree (genesis4_cuda_port_amd/_lib/).
+
+Every source is compiled with hipcc for gfx950 only.  Packed FP32 vector ops
+are disabled (-packed-fp32-ops) and hardware FP atomics are requested
+(-munsafe-fp-atomics) for the deposition's global fallback adds.
+
+    python -m genesis4_cuda_port_amd.build          # build
+    python -m genesis4_cuda_port_amd.build --clean  # remove objects first
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "_lib"
+OBJ = PKG / "_build"
+LIB = OUT / "libg4hip.so"
+
+SOURCES = [
+    "particle.hip",
+    "field_fft.hip",
+    "field_adi.hip",
+    "field_misc.hip",
+    "abi.cpp",
+    "run.cpp",
+    "frontend.cpp",
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = [
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-Wall",
+    "-Wno-unused-result",
+]
+DEVICE = [
+    f"--offload-arch={ARCH}",
+    "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _compile(src: str) -> Path:
+    path = CSRC / src
+    obj = OBJ / (src + ".o")
+    cmd = [HIPCC, *COMMON]
+    if src.endswith(".hip"):
+        cmd += ["-x", "hip", *DEVICE]
+    else:
+        cmd += ["-D__HIP_PLATFORM_AMD__"]
+    cmd += ["-c", str(path), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
+def build(clean: bool = False, jobs: int | None = None) -> Path:
+    if clean and OBJ.exists():
+        shutil.rmtree(OBJ)
+    OBJ.mkdir(parents=True, exist_ok=True)
+    OUT.mkdir(parents=True, exist_ok=True)
+    jobs = jobs or min(8, len(SOURCES))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    """Compile the FP64 CPU oracle (test infrastructure) with its Makefile."""
+    odir = PKG.parent / "oracle"
+    res = subprocess.run(["make", "-C", str(odir), "-B"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return odir / "_build" / "libg4oracle.so"
+
+
+if __name__ == "__main__":
+    lib = build(clean="--clean" in sys.argv)
+    print(lib)

@@ -1,0 +1,379 @@
+// field_adi.hip - ADI paraxial step (FieldSolverADICUDA.cpp:176-380,
+// 789-816, 821-866) as two Thomas sweeps in FP64 for both precisions.
+//
+// Step A: r = u + i rtmp d2y(u) + S, solve (1 - i rtmp d2x) u* = r along rows.
+// Step B: r = u* + i rtmp d2x(u*) + S, solve (1 - i rtmp d2y) u = r along
+// columns.  Missing neighbours at the grid edge are zero (the reference's
+// edge rule).  Thomas coefficients c, cbet, cwet are the reference's
+// (FieldSolverADICUDA.cpp:821-866), built once per (delz, ngrid) on the host.
+//
+// One workgroup owns one ring plane, so the sweeps update the plane in place.
+//  - x sweep: blocks of kB columns of all rows are staged through LDS with
+//    coalesced row segments; each thread runs the recurrence of one row; the
+//    forward values go to an FP64 scratch plane and come back for the
+//    backward substitution.
+//  - y sweep: each thread runs the recurrence of one column; rows are read
+//    coalesced in chunks of kR rows (one memory latency per chunk), the x
+//    neighbours come from an LDS row buffer.  The backward pass of the y
+//    sweep produces the final field and reduces the field diagnostics of the
+//    plane when asked.
+// The RHS, the source add and the Thomas recurrences are FP64 in both
+// precisions; the stored field is FP32 in MIXED.
+#include "kernels.hpp"
+
+#include <complex>
+#include <vector>
+
+namespace g4 {
+namespace {
+
+constexpr int kB = 8;   // columns per x-sweep block
+constexpr int kR = 8;   // rows per y-sweep chunk
+
+template <class P>
+struct A2;
+template <>
+struct A2<double> {
+  using T = double2;
+};
+template <>
+struct A2<float> {
+  using T = float2;
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// rhs = center + i*cs*((a + b) - 2 center) [+ s]
+__device__ __forceinline__ double2 rhs_of(double2 center, double2 a, double2 b, double cs) {
+  const double2 lap = zsub(zadd(a, b), make_double2(center.x * 2.0, center.y * 2.0));
+  return make_double2(center.x + -cs * lap.y, center.y + cs * lap.x);
+}
+
+template <class T2>
+__device__ __forceinline__ double2 to_d(T2 v) {
+  return make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+}
+
+template <class P>
+__global__ __launch_bounds__(1024) void adi_x_kernel(typename A2<P>::T* u, double2* ys, int nslice,
+                                                     int G, double cs, const double2* c_d,
+                                                     const double2* cbet_d, const double2* cwet_d,
+                                                     const typename A2<P>::T* src, const int4* rect,
+                                                     int first, int add_src) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 sm[];
+  double2* cl = sm;
+  double2* cb = sm + G;
+  double2* cw = sm + 2 * G;
+  double2* tile = sm + 3 * G;  // (G+2) rows x kB
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
+  const int nt = blockDim.x;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  T2* up = u + p * plane;
+  double2* yp = ys + p * plane;
+  for (int k = t; k < G; k += nt) {
+    cl[k] = c_d[k];
+    cb[k] = cbet_d[k];
+    cw[k] = cwet_d[k];
+  }
+  for (int k = t; k < kB; k += nt) {
+    tile[k] = make_double2(0.0, 0.0);
+    tile[(G + 1) * kB + k] = make_double2(0.0, 0.0);
+  }
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const int nblk = (G + kB - 1) / kB;
+  double2 yprev = make_double2(0.0, 0.0);
+  for (int b = 0; b < nblk; ++b) {
+    const int c0 = b * kB, nb = min(kB, G - c0);
+    __syncthreads();
+    for (int e = t; e < G * nb; e += nt) {
+      const int iy = e / nb, c = e - iy * nb;
+      tile[(iy + 1) * kB + c] = to_d(up[static_cast<int64_t>(iy) * G + c0 + c]);
+    }
+    __syncthreads();
+    double2 yv[kB];
+    if (t < G) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c) {
+        if (c < nb) {
+          const int k = c0 + c;
+          const double2 center = tile[(t + 1) * kB + c];
+          double2 r = rhs_of(center, tile[(t + 2) * kB + c], tile[t * kB + c], cs);
+          if (sp && k >= rc.x && k < rc.z && t >= rc.y && t < rc.w)
+            r = zadd(r, to_d(sp[static_cast<int64_t>(t) * G + k]));
+          yprev = zmul(zsub(r, zmul(cl[k], yprev)), cb[k]);
+          yv[c] = yprev;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < G) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c)
+        if (c < nb) tile[(t + 1) * kB + c] = yv[c];
+    }
+    __syncthreads();
+    for (int e = t; e < G * nb; e += nt) {
+      const int iy = e / nb, c = e - iy * nb;
+      yp[static_cast<int64_t>(iy) * G + c0 + c] = tile[(iy + 1) * kB + c];
+    }
+  }
+  // backward substitution, right to left
+  double2 unext = make_double2(0.0, 0.0);
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int c0 = b * kB, nb = min(kB, G - c0);
+    __syncthreads();
+    for (int e = t; e < G * nb; e += nt) {
+      const int iy = e / nb, c = e - iy * nb;
+      tile[(iy + 1) * kB + c] = yp[static_cast<int64_t>(iy) * G + c0 + c];
+    }
+    __syncthreads();
+    if (t < G) {
+#pragma unroll
+      for (int c = kB - 1; c >= 0; --c) {
+        if (c < nb) {
+          const int k = c0 + c;
+          const double2 y = tile[(t + 1) * kB + c];
+          const double2 uu = (k == G - 1) ? y : zsub(y, zmul(cw[k + 1], unext));
+          tile[(t + 1) * kB + c] = uu;
+          unext = uu;
+        }
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < G * nb; e += nt) {
+      const int iy = e / nb, c = e - iy * nb;
+      const double2 v = tile[(iy + 1) * kB + c];
+      T2 o;
+      o.x = static_cast<P>(v.x);
+      o.y = static_cast<P>(v.y);
+      up[static_cast<int64_t>(iy) * G + c0 + c] = o;
+    }
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, double2* ys, int nslice,
+                                                     int G, double cs, const double2* c_d,
+                                                     const double2* cbet_d, const double2* cwet_d,
+                                                     const typename A2<P>::T* src, const int4* rect,
+                                                     int first, int add_src, double* stats) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 sm[];
+  double2* cl = sm;
+  double2* cb = sm + G;
+  double2* cw = sm + 2 * G;
+  double2* rb = sm + 3 * G;  // 2 x (G+2) row buffers
+  __shared__ double red[16][8];
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
+  const int nt = blockDim.x;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  T2* up = u + p * plane;
+  double2* yp = ys + p * plane;
+  for (int k = t; k < G; k += nt) {
+    cl[k] = c_d[k];
+    cb[k] = cbet_d[k];
+    cw[k] = cwet_d[k];
+  }
+  if (t < 2) {
+    rb[t * (G + 2)] = make_double2(0.0, 0.0);
+    rb[t * (G + 2) + G + 1] = make_double2(0.0, 0.0);
+  }
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const bool col = t < G;
+  const bool src_col = sp && t >= rc.x && t < rc.z;
+  double2 yprev = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < G; k0 += kR) {
+    double2 v[kR];
+    double2 sv[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const int k = k0 + r;
+      v[r] = make_double2(0.0, 0.0);
+      sv[r] = make_double2(0.0, 0.0);
+      if (col && k < G) {
+        v[r] = to_d(up[static_cast<int64_t>(k) * G + t]);
+        if (src_col && k >= rc.y && k < rc.w) sv[r] = to_d(sp[static_cast<int64_t>(k) * G + t]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const int k = k0 + r;
+      if (k < G) {  // uniform
+        double2* buf = rb + (k & 1) * (G + 2);
+        if (col) buf[t + 1] = v[r];
+        __syncthreads();
+        if (col) {
+          double2 rr = rhs_of(v[r], buf[t + 2], buf[t], cs);
+          if (src_col && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
+          yprev = zmul(zsub(rr, zmul(cl[k], yprev)), cb[k]);
+          yp[static_cast<int64_t>(k) * G + t] = yprev;
+        }
+      }
+    }
+  }
+  // backward substitution, bottom to top; field diagnostics of the plane
+  const double shift = -0.5 * static_cast<double>(G - 1);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int64_t icenter = (plane - 1) / 2;
+  double2 unext = make_double2(0.0, 0.0);
+  const int nchunk = (G + kR - 1) / kR;
+  for (int ch = nchunk - 1; ch >= 0; --ch) {
+    const int k0 = ch * kR;
+    double2 y[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const int k = k0 + r;
+      y[r] = (col && k < G) ? yp[static_cast<int64_t>(k) * G + t] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int r = kR - 1; r >= 0; --r) {
+      const int k = k0 + r;
+      if (col && k < G) {
+        const double2 uu = (k == G - 1) ? y[r] : zsub(y[r], zmul(cw[k + 1], unext));
+        unext = uu;
+        T2 o;
+        o.x = static_cast<P>(uu.x);
+        o.y = static_cast<P>(uu.y);
+        const int64_t gi = static_cast<int64_t>(k) * G + t;
+        up[gi] = o;
+        if (stats) {
+          const double re = o.x, im = o.y;
+          const double wei = re * re + im * im;
+          const double dx = t + shift, dy = k + shift;
+          acc[0] += wei;
+          acc[1] += dx * wei;
+          acc[2] += dx * dx * wei;
+          acc[3] += dy * wei;
+          acc[4] += dy * dy * wei;
+          acc[5] += re;
+          acc[6] += im;
+          if (gi == icenter) {
+            stats[p * G4H_FIELD_RAW + 7] = re;
+            stats[p * G4H_FIELD_RAW + 8] = im;
+          }
+        }
+      }
+    }
+  }
+  if (stats) {
+    const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      double v = acc[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (t < 7) {
+      double v = 0.0;
+      for (int w = 0; w < nt / 64; ++w) v += red[w][t];
+      stats[p * G4H_FIELD_RAW + t] = v;
+    }
+  }
+}
+
+template <class P>
+void adi_step(Field* f, bool source_on, bool stats) {
+  using T2 = typename A2<P>::T;
+  const int G = f->ngrid;
+  require(G <= 360, "ADI solver supports ngrid <= 360 (LDS budget of the x sweep)");
+  const int nt = ((G + 63) / 64) * 64;
+  const double cs = 0.25 * f->adi_delz / (f->xks * f->dgrid * f->dgrid);
+  if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
+  T2* u = static_cast<T2*>(f->data());
+  const T2* src = static_cast<const T2*>(f->src());
+  const size_t ldsx = sizeof(double2) * (3 * static_cast<size_t>(G) + (G + 2) * static_cast<size_t>(kB));
+  const size_t ldsy = sizeof(double2) * (3 * static_cast<size_t>(G) + 2 * static_cast<size_t>(G + 2));
+  const int add = source_on ? 1 : 0;
+  f->ctx->launch(G4H_K_ADI_X, [&] {
+    hipLaunchKernelGGL(adi_x_kernel<P>, dim3(f->nslice), dim3(nt), ldsx, f->ctx->stream, u,
+                       f->adi_y.get(), f->nslice, G, cs, f->adi_c.get(), f->adi_cbet.get(),
+                       f->adi_cwet.get(), src, f->src_rect.get(), f->first, add);
+  });
+  f->ctx->launch(G4H_K_ADI_Y, [&] {
+    hipLaunchKernelGGL(adi_y_kernel<P>, dim3(f->nslice), dim3(nt), ldsy, f->ctx->stream, u,
+                       f->adi_y.get(), f->nslice, G, cs, f->adi_c.get(), f->adi_cbet.get(),
+                       f->adi_cwet.get(), src, f->src_rect.get(), f->first, add,
+                       stats ? f->plane_stats.get() : nullptr);
+  });
+}
+
+}  // namespace
+
+// Thomas factors of FieldSolverADICUDA::init (FieldSolverADICUDA.cpp:821-866).
+void adi_prepare(Field* f, double delz) {
+  const int G = f->ngrid;
+  if (delz == f->adi_delz && G == f->adi_ngrid && f->adi_c.size() == static_cast<size_t>(G)) return;
+  const double rtmp = 0.25 * delz / (f->xks * f->dgrid * f->dgrid);
+  using cd = std::complex<double>;
+  std::vector<double> mupp(G), mmid(G), mlow(G);
+  mupp[0] = rtmp;
+  mmid[0] = -2 * rtmp;
+  mlow[0] = 0;
+  for (int i = 1; i < G - 1; i++) {
+    mupp[i] = rtmp;
+    mmid[i] = -2 * rtmp;
+    mlow[i] = rtmp;
+  }
+  mupp[G - 1] = 0;
+  mmid[G - 1] = -2 * rtmp;
+  mlow[G - 1] = rtmp;
+  std::vector<cd> w1(G), w2(G), c(G), cbet(G), cwet(G);
+  for (int i = 0; i < G; i++) {
+    w1[i] = cd(0.0, -mupp[i]);
+    w2[i] = cd(1.0, -mmid[i]);
+    c[i] = cd(0.0, -mlow[i]);
+  }
+  cbet[0] = cd(1.0, 0.0) / w2[0];
+  cwet[0] = cd(0.0, 0.0);
+  for (int i = 1; i < G; i++) {
+    cwet[i] = w1[i - 1] * cbet[i - 1];
+    cbet[i] = cd(1.0, 0.0) / (w2[i] - c[i] * cwet[i]);
+  }
+  std::vector<double2> hc(G), hb(G), hw(G);
+  for (int i = 0; i < G; ++i) {
+    hc[i] = make_double2(c[i].real(), c[i].imag());
+    hb[i] = make_double2(cbet[i].real(), cbet[i].imag());
+    hw[i] = make_double2(cwet[i].real(), cwet[i].imag());
+  }
+  if (f->adi_c.size() != static_cast<size_t>(G)) {
+    f->adi_c.alloc(f->ctx, G);
+    f->adi_cbet.alloc(f->ctx, G);
+    f->adi_cwet.alloc(f->ctx, G);
+  }
+  G4_HIP(hipMemcpyAsync(f->adi_c.get(), hc.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
+  G4_HIP(hipMemcpyAsync(f->adi_cbet.get(), hb.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
+  G4_HIP(hipMemcpyAsync(f->adi_cwet.get(), hw.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
+  G4_HIP(hipStreamSynchronize(f->ctx->stream));
+  f->adi_delz = delz;
+  f->adi_ngrid = G;
+}
+
+void adi_solve(Field* f, bool source_on, bool stats) {
+  if (f->prec == G4H_FP64)
+    adi_step<double>(f, source_on, stats);
+  else
+    adi_step<float>(f, source_on, stats);
+}
+
+}  // namespace g4

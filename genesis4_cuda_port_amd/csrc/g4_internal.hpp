@@ -1,0 +1,211 @@
+// g4_internal.hpp - internal state behind the g4hip C ABI.
+//
+// Device layout (all in HBM, one allocation per array):
+//   beam  FP64 : x,y,px,py,gamma,theta  double[N]
+//   beam  MIXED: x,y,px,py,dgamma float[N] (dgamma = gamma - gamma_ref), theta double[N]
+//   field       : interleaved complex (double2 / float2) [nslice*ngrid*ngrid],
+//                 plane p = ring slot p, cell ix + iy*ngrid
+//   source      : same element type, indexed by BEAM slice (not ring slot), with a
+//                 per-slice half-open rectangle outside which it is zero
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/g4hip.h"
+
+namespace g4 {
+
+// Error carried to the ABI boundary and turned into a status code there.
+struct Error : std::runtime_error {
+  g4h_status code;
+  Error(g4h_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(g4h_status c, const std::string& m) { throw Error(c, m); }
+
+inline void require(bool ok, const char* what) {
+  if (!ok) fail(G4H_ERR_INVALID_ARGUMENT, what);
+}
+
+inline void hip_check(hipError_t e, const char* where) {
+  if (e != hipSuccess) {
+    const g4h_status c = (e == hipErrorOutOfMemory) ? G4H_ERR_OUT_OF_MEMORY : G4H_ERR_DEVICE;
+    fail(c, std::string(where) + ": " + hipGetErrorString(e));
+  }
+}
+#define G4_HIP(expr) ::g4::hip_check((expr), #expr)
+
+void set_last_error(const std::string& m);
+
+// Per-context byte accounting so a caller (or a serving loop) can admit work
+// against an HBM budget.
+struct Context;
+
+// RAII device allocation of `n` elements of T.
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { swap(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      swap(o);
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+
+  void alloc(Context* ctx, size_t n);
+  void release();
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+
+ private:
+  void swap(DevBuf& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    std::swap(ctx_, o.ctx_);
+  }
+  T* p_ = nullptr;
+  size_t n_ = 0;
+  Context* ctx_ = nullptr;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t bytes = 0;
+  // profiling
+  bool profile = false;
+  struct Prof {
+    int64_t launches = 0;
+    double ms = 0.0;
+  } prof[G4H_K_COUNT];
+  struct Pending {
+    hipEvent_t a, b;
+    int cls;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  void activate() const { G4_HIP(hipSetDevice(device)); }
+  hipEvent_t get_event();
+  // Bracket a launch with events when profiling is on.
+  template <class F>
+  void launch(int cls, F&& f) {
+    if (!profile) {
+      f();
+      G4_HIP(hipGetLastError());
+      return;
+    }
+    Pending p{get_event(), get_event(), cls};
+    G4_HIP(hipEventRecord(p.a, stream));
+    f();
+    G4_HIP(hipGetLastError());
+    G4_HIP(hipEventRecord(p.b, stream));
+    pending.push_back(p);
+    if (pending.size() > 4096) collect();
+  }
+  void collect();  // synchronises and folds pending events into prof[]
+};
+
+template <class T>
+void DevBuf<T>::alloc(Context* ctx, size_t n) {
+  release();
+  if (n == 0) return;
+  void* p = nullptr;
+  G4_HIP(hipMalloc(&p, n * sizeof(T)));
+  p_ = static_cast<T*>(p);
+  n_ = n;
+  ctx_ = ctx;
+  if (ctx_) ctx_->bytes += n * sizeof(T);
+}
+
+template <class T>
+void DevBuf<T>::release() {
+  if (p_) {
+    (void)hipFree(p_);
+    if (ctx_) ctx_->bytes -= n_ * sizeof(T);
+  }
+  p_ = nullptr;
+  n_ = 0;
+  ctx_ = nullptr;
+}
+
+struct Beam {
+  Context* ctx = nullptr;
+  g4h_precision prec = G4H_FP64;
+  int nslice = 0;
+  int64_t np = 0;
+  double gamma_ref = 0.0;
+  bool uploaded = false;
+  std::vector<int32_t> off_h;
+  std::vector<double> current_h;
+  DevBuf<int32_t> off;           // [nslice+1]
+  DevBuf<double> cur_over_npar;  // [nslice]  current/npar (0 if npar == 0)
+  // state
+  DevBuf<double> xd, yd, pxd, pyd, gd;  // FP64
+  DevBuf<float> xf, yf, pxf, pyf, gf;   // MIXED (gf = gamma - gamma_ref)
+  DevBuf<double> theta;                 // both
+  DevBuf<double> eloss;                 // per-slice longitudinal field (optional)
+  int max_slice_np = 0;
+};
+
+struct Field {
+  Context* ctx = nullptr;
+  g4h_precision prec = G4H_FP64;
+  int nslice = 0, ngrid = 0, harm = 1;
+  double gridmax = 0, dgrid = 0, xlambda = 0, xks = 0, slicelength = 0;
+  int first = 0;
+  double accuslip = 0.0;
+  bool enabled = true;
+  bool uploaded = false;
+  // storage
+  DevBuf<double2> ud;  // FP64 planes
+  DevBuf<float2> uf;   // MIXED planes
+  // source (beam-slice order) + bookkeeping for the windowed deposition
+  DevBuf<double2> srcd;
+  DevBuf<float2> srcf;
+  DevBuf<int4> src_rect;  // [nslice] half-open (x0,y0,x1,y1) of possibly nonzero cells
+  DevBuf<int4> dep_bbox;  // [nslice] (minx,miny,maxx,maxy) of deposited lower-left cells
+  // solver
+  g4h_solver_kind kind = G4H_SOLVER_ADI;
+  bool filter = false;
+  double xc = 1, yc = 1, sig = 1;
+  bool solver_ready = false;
+  // ADI
+  double adi_delz = 0.0;
+  int adi_ngrid = 0;
+  DevBuf<double2> adi_c, adi_cbet, adi_cwet;  // [ngrid]
+  DevBuf<double2> adi_y;                      // FP64 intermediates [nslice*ngrid^2]
+  // FFT
+  double fft_delz = 0.0;
+  DevBuf<double2> fft_tw;    // exp(-2 pi i k / G), [G]
+  DevBuf<double2> fft_prop;  // separable propagator factor [G]
+  DevBuf<double> fft_sigmoid;  // [G*G] (filter only)
+  DevBuf<double2> fft_work;    // filtered-source work planes (filter only)
+  // diagnostics
+  DevBuf<double> plane_stats;  // [nslice * G4H_FIELD_RAW] per ring slot (last solve)
+  DevBuf<double> col_partials; // FFT column-tile partial sums
+  DevBuf<int32_t> zeroed;      // [nslice] ring slots zeroed by slippage since last map
+  bool stats_valid = false;
+
+  int64_t cells() const { return static_cast<int64_t>(nslice) * ngrid * ngrid; }
+  void* data() const { return prec == G4H_FP64 ? static_cast<void*>(ud.get()) : static_cast<void*>(uf.get()); }
+  void* src() const { return prec == G4H_FP64 ? static_cast<void*>(srcd.get()) : static_cast<void*>(srcf.get()); }
+};
+
+}  // namespace g4
+
+struct g4h_context : g4::Context {};
+struct g4h_beam : g4::Beam {};
+struct g4h_field : g4::Field {};

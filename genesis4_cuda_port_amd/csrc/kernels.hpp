@@ -1,0 +1,99 @@
+// kernels.hpp - parameter blocks and host launchers of the gfx950 kernels.
+#pragma once
+
+#include "g4_internal.hpp"
+
+namespace g4 {
+
+// ---- particle pass -------------------------------------------------------
+enum Stage : int {
+  ST_T1 = 1,     // first transverse half step (with chicane)
+  ST_PUSH = 2,   // gather + RK4
+  ST_R56 = 4,    // chicane R56 phase shift
+  ST_T2 = 8,     // last transverse half step (with corrector)
+  ST_DEP = 16,   // source deposition
+  ST_DIAG = 32   // per-slice beam sums
+};
+
+// One transverse half step: TrackBeamCUDA::track (TrackBeamCUDA.cpp:156-204).
+struct TransParams {
+  double m00, m01, m10, m11, m22, m23, m32, m33;  // chicane matrix
+  double cx, cy;                                   // corrector kick * gamma0
+  double xstr, xoff, ystr, yoff;                   // plane maps (offset already / strength)
+  double delz;                                     // half step
+  double aw2;                                      // aw^2 of the step
+  double g2c;      // gamma0^2 - 1 - aw^2   (mixed gammaz)
+  double g2chic;   // gamma0^2 - 1          (mixed chicane gammaz)
+  int xmode, ymode;  // 0 drift, 1 focusing, 2 defocusing
+  int chicane, corrector;
+};
+
+struct GatherField {
+  const void* data;  // double2* or float2*, ring storage
+  double gridmax, dgrid, rtmp, rharm;
+  int first, ngrid, nslice, pad_;
+};
+
+constexpr int kMaxFields = 8;
+
+struct PushParams {
+  double xks, xku, aw, autophase, delz, ax, ay, kx, ky, gradx, grady;
+  const double* slice_eloss;  // device [nslice] or null
+  // mixed-precision phase equation: theta' = -(c0 + eta*(k1 + k2*eta))
+  double c0, k1, k2, inv_b0;
+  int nfield, pad_;
+  GatherField f[kMaxFields];
+};
+
+struct DepParams {
+  void* src;                    // beam-slice order, double2*/float2*
+  int4* rect;                   // [nslice]
+  int4* bbox;                   // [nslice]
+  const double* cur_over_npar;  // [nslice]
+  double scl_factor;            // vacimp*xks*fc*delz/(4*eev*dgrid^2)
+  double gridmax, dgrid;
+  double ax, ay, kx, ky, gradx, grady;
+  int ngrid, harm;
+  int window_cap;  // cells of the LDS window
+  int pad_;
+};
+
+struct DiagParams {
+  double* raw;  // [nslice * G4H_BEAM_RAW(nharm)]
+  int nharm, pad_;
+};
+
+struct ParticleParams {
+  int stages, pad_;
+  double gamma0;  // beam reference energy (mixed encoding, R56)
+  double r56;     // R56*2pi/lambda0/gamma0
+  TransParams t1, t2;
+  PushParams push;
+  DepParams dep;
+  DiagParams diag;
+};
+
+void launch_particle(Context* ctx, Beam* b, const ParticleParams& p);
+void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell);
+
+// Host construction of parameter blocks from a lattice step.
+TransParams make_trans(const g4h_step& st, double gamma0, double delz, bool last);
+double make_r56(const g4h_step& st, double gamma0, double lambda0);
+void make_push(PushParams& p, const g4h_step& st, double gamma0, double delz,
+               Field* const* fields, int nfield, const double* d_eloss);
+bool make_dep(DepParams& d, Field* f, Beam* b, const g4h_step& st, double delz);
+
+// ---- field solvers -------------------------------------------------------
+void field_prepare_solver(Field* f, double delz);  // (re)build coefficient tables
+// One field step with the source (if source_on).  When stats != 0 the last
+// pass also writes per-ring-slot field sums into f->plane_stats.
+void field_solve(Field* f, bool source_on, bool stats);
+void field_plane_stats(Field* f);                      // standalone field sums
+void field_zero_plane(Field* f, int slot);             // slippage zero fill
+// plane_stats (ring order) -> raw[nslice*9] (logical order), honouring zeroed slots.
+void field_map_stats(Field* f, double* d_raw);
+// source bookkeeping reset (new beam or new source allocation)
+void field_reset_source(Field* f);
+void ensure_source(Field* f);
+
+}  // namespace g4

@@ -1,0 +1,704 @@
+// particle.hip - the fused particle pass for gfx950.
+//
+// One workgroup owns one beam slice (particles [off[s], off[s+1]) are
+// contiguous) and runs, per particle and as selected by ParticleParams.stages:
+//   first transverse half step  (TrackBeamCUDA.cpp:156-204, chicane :220-255)
+//   bilinear gather + RK4 push  (BeamSolverCUDA.cpp:87-304, multi-field :306-409)
+//   R56                          (TrackBeamCUDA.cpp:257-278)
+//   last transverse half step   (corrector :206-218)
+//   source deposition           (FieldSolverADICUDA.cpp:109-174)
+//   per-slice beam sums          (DiagnosticCUDA.cpp:188-303)
+// so the particle state crosses HBM once per step (read + write).
+//
+// Deposition: the workgroup accumulates into an LDS window (FP64 complex
+// accumulators, ds_add_f64) placed on the slice's deposition footprint of the
+// previous step plus a 2-cell margin; cells outside the window go straight to
+// the source plane with global atomics.  After the slice the window is
+// written with plain stores (the workgroup owns the slice's source plane), so
+// each touched cell costs one global write instead of one atomic per
+// particle.  FP64 accumulation of the per-particle terms is the compensated
+// sum: it carries 29 more bits than an FP32 sum, which is what a Kahan
+// compensation term would add, at the cost of one LDS atomic.
+//
+// MIXED arithmetic: x, y, px, py and dgamma = gamma - gamma_ref in FP32,
+// theta in FP64.  The phase equation theta' = xks(1 - 1/beta_par) + xku is
+// evaluated around the reference particle,
+//   theta' = -(c0 + k1*eta + k2*eta^2),  eta = eps/eps0 - 1,
+//   eps = btper0/gamma^2, eps0 = (1+aw^2)/gamma_ref^2,
+// with c0, k1, k2 from FP64 on the host; eta is small and well conditioned in
+// FP32, so FP32 keeps ~1e-9 rad/m absolute accuracy where the direct form
+// would cancel 240 - 240 in FP32.
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace g4 {
+namespace {
+
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxHarm = 8;
+constexpr int kNSum = 13 + 2 * kMaxHarm;  // sums of the beam diagnostics
+constexpr int kNMinMax = 10;
+
+template <class P>
+struct V2;
+template <>
+struct V2<double> {
+  using T = double2;
+};
+template <>
+struct V2<float> {
+  using T = float2;
+};
+
+template <class P>
+struct BeamPtr {
+  P *x, *y, *px, *py, *g;  // g: gamma (FP64) or gamma - gamma_ref (MIXED)
+  double* th;
+};
+
+// theta reduced to [-pi, pi] in FP64, then FP32 sine and cosine.
+__device__ __forceinline__ void sincos_red(double a, float& s, float& c) {
+  const double n = rint(a * 0.15915494309189535);
+  double r = fma(-n, 6.283185307179586, a);
+  r = fma(-n, 2.4492935982947064e-16, r);
+  sincosf(static_cast<float>(r), &s, &c);
+}
+
+// ---- transverse ---------------------------------------------------------
+template <class P>
+__device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& coord, P& mom,
+                                           P gz) {
+  if (mode == 0) {
+    coord += mom * delz / gz;
+    return;
+  }
+  if (mode == 1) {
+    const P foc = sqrt(str / gz);
+    const P omg = foc * delz;
+    const P a1 = cos(omg);
+    const P a2 = sin(omg) / foc;
+    const P a3 = -a2 * foc * foc;
+    const P xt = coord - off;
+    coord = a1 * xt + a2 * mom / gz + off;
+    mom = a3 * xt * gz + a1 * mom;
+    return;
+  }
+  const P foc = sqrt(-str / gz);
+  const P omg = foc * delz;
+  const P a1 = cosh(omg);
+  const P a2 = sinh(omg) / foc;
+  const P a3 = a2 * foc * foc;
+  const P xt = coord - off;
+  coord = a1 * xt + a2 * mom / gz + off;
+  mom = a3 * xt * gz + a1 * mom;
+}
+
+// gamma^2 - 1 - extra (extra = aw^2 + px^2 + py^2 or px^2 + py^2)
+__device__ __forceinline__ double gz2(double g, double g0, double g2c, double aw2, bool with_aw) {
+  (void)g0;
+  (void)g2c;
+  return with_aw ? g * g - 1.0 - aw2 : g * g - 1.0;
+}
+__device__ __forceinline__ float gz2(float dg, double g0, double g2c, double aw2, bool with_aw) {
+  (void)aw2;
+  (void)with_aw;
+  return static_cast<float>(g2c) + dg * (2.0f * static_cast<float>(g0) + dg);
+}
+
+template <class P>
+__device__ __forceinline__ void transverse(const TransParams& t, double g0, P g, P& x, P& y, P& px,
+                                           P& py, bool first_half) {
+  if (first_half && t.chicane) {
+    const P gz = sqrt(gz2(g, g0, t.g2chic, 0.0, false) - px * px - py * py);
+    P tmp = x;
+    x = P(t.m00) * tmp + P(t.m01) * px / gz;
+    px = P(t.m10) * tmp * gz + P(t.m11) * px;
+    tmp = y;
+    y = P(t.m22) * tmp + P(t.m23) * py / gz;
+    py = P(t.m32) * tmp * gz + P(t.m33) * py;
+  }
+  if (!first_half && t.corrector) {
+    px += P(t.cx);
+    py += P(t.cy);
+  }
+  const P gz = sqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
+  plane_step<P>(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
+  plane_step<P>(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
+}
+
+// ---- particle -> cell (BeamSolverCUDA.cpp:99-116, FieldSolverADICUDA.cpp:
+// 126-143): lower-left cell (ix, iy) and the weight wx, wy of that cell's
+// column/row.  The gather, the deposition and g4h_beam_cell_index share it.
+template <class P>
+__device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, int ng, int& ix, int& iy, P& wx,
+                                        P& wy) {
+  if ((x <= -gm) || (x >= gm) || (y <= -gm) || (y >= gm)) return false;
+  const P wxr = (x + gm) / dg;
+  const P wyr = (y + gm) / dg;
+  ix = static_cast<int>(floor(wxr));
+  iy = static_cast<int>(floor(wyr));
+  if (ix < 0 || ix >= ng - 1 || iy < 0 || iy >= ng - 1) return false;
+  wx = P(1) + static_cast<P>(ix) - wxr;
+  wy = P(1) + static_cast<P>(iy) - wyr;
+  return true;
+}
+
+// ---- gather (BeamSolverCUDA.cpp:87-143) ---------------------------------
+template <class P>
+__device__ __forceinline__ void gather(const GatherField& f, int slice, P x, P y, P awloc, P& rr,
+                                       P& ri) {
+  using T2 = typename V2<P>::T;
+  rr = P(0);
+  ri = P(0);
+  const int ng = f.ngrid;
+  int ix, iy;
+  P wx, wy;
+  if (!cell_of<P>(x, y, P(f.gridmax), P(f.dgrid), ng, ix, iy, wx, wy)) return;
+  const int islice = (slice + f.first) % f.nslice;
+  const T2* u = static_cast<const T2*>(f.data) + static_cast<int64_t>(islice) * ng * ng +
+                ix + static_cast<int64_t>(iy) * ng;
+  const P w00 = wx * wy;
+  const P w10 = (P(1) - wx) * wy;
+  const P w01 = wx * (P(1) - wy);
+  const P w11 = (P(1) - wx) * (P(1) - wy);
+  const T2 a = u[0], b = u[1], c = u[ng], d = u[ng + 1];
+  P cre = a.x * w00;
+  P cim = a.y * w00;
+  cre += b.x * w10;
+  cim += b.y * w10;
+  cre += c.x * w01;
+  cim += c.y * w01;
+  cre += d.x * w11;
+  cim += d.y * w11;
+  const P scale = P(f.rtmp) * awloc;
+  rr = scale * cre;
+  ri = -scale * cim;
+}
+
+// ---- ODE ----------------------------------------------------------------
+// FP64: evaluate_longitudinal_ode_* (BeamSolverCUDA.cpp:145-223) verbatim.
+template <int NF>
+__device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, double tthet,
+                                      double btpar, double ez, const double* rr, const double* ri,
+                                      double& k2gg, double& k2pp) {
+  const double ztemp1 = -2.0 / p.xks;
+  double cre = 0.0, cim = 0.0;
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    double s, c;
+    sincos(p.f[i].rharm * tthet, &s, &c);
+    cre += rr[i] * c + ri[i] * s;
+    cim += ri[i] * c - rr[i] * s;
+  }
+  const double btper0 = btpar + ztemp1 * cre;
+  const double btpar0 = sqrt(1.0 - btper0 / (tgam * tgam));
+  k2pp += p.xks * (1.0 - 1.0 / btpar0) + p.xku;
+  k2gg += cim / btpar0 / tgam - ez;
+}
+
+// MIXED: dgamma (FP32) and theta (FP64) with the expansion of the header.
+struct MixedK {
+  float ztemp1, c0, k1, k2, inv_b0, inv_g0;
+};
+
+template <int NF>
+__device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int nf, float dg,
+                                      double tthet, float db0, float ez, const float* rr,
+                                      const float* ri, float& k2gg, float& k2pp) {
+  float cre = 0.0f, cim = 0.0f;
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    float s, c;
+    sincos_red(p.f[i].rharm * tthet, s, c);
+    cre += rr[i] * c + ri[i] * s;
+    cim += ri[i] * c - rr[i] * s;
+  }
+  const float db = db0 + m.ztemp1 * cre;
+  const float d = dg * m.inv_g0;
+  const float opd = 1.0f + d;
+  const float eta = (db * m.inv_b0 - d * (2.0f + d)) / (opd * opd);
+  k2pp += -(m.c0 + eta * (m.k1 + m.k2 * eta));
+  k2gg += cim * m.inv_g0 / opd - ez;
+}
+
+// RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
+template <int NF>
+__device__ __forceinline__ void push64(const PushParams& p, int slice, double x, double y, double px,
+                                       double py, double& gamma, double& theta) {
+  const int nf = NF > 0 ? NF : p.nfield;
+  double g = gamma;
+  double t = theta + p.autophase;
+  const double dx = x - p.ax, dy = y - p.ay;
+  const double awloc = 1.0 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy;
+  const double btpar = 1.0 + px * px + py * py + p.aw * p.aw * awloc * awloc;
+  const double ez = p.slice_eloss ? p.slice_eloss[slice] : 0.0;
+  double rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    gather<double>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
+  }
+  double k2gg = 0.0, k2pp = 0.0, k3gg, k3pp;
+  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  double stpz = 0.5 * p.delz;
+  g += stpz * k2gg;
+  t += stpz * k2pp;
+  k3gg = k2gg;
+  k3pp = k2pp;
+  k2gg = 0.0;
+  k2pp = 0.0;
+  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  g += stpz * (k2gg - k3gg);
+  t += stpz * (k2pp - k3pp);
+  k3gg /= 6.0;
+  k3pp /= 6.0;
+  k2gg *= -0.5;
+  k2pp *= -0.5;
+  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  stpz = p.delz;
+  g += stpz * k2gg;
+  t += stpz * k2pp;
+  k3gg -= k2gg;
+  k3pp -= k2pp;
+  k2gg *= 2.0;
+  k2pp *= 2.0;
+  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  g += stpz * (k3gg + k2gg / 6.0);
+  t += stpz * (k3pp + k2pp / 6.0);
+  gamma = g;
+  theta = t;
+}
+
+template <int NF>
+__device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int slice, float x,
+                                       float y, float px, float py, float& dgamma, double& theta) {
+  const int nf = NF > 0 ? NF : p.nfield;
+  float g = dgamma;
+  double t = theta + p.autophase;
+  const float dx = x - float(p.ax), dy = y - float(p.ay);
+  const float awm1 = 0.5f * (float(p.kx) * dx * dx + float(p.ky) * dy * dy) + float(p.gradx) * dx +
+                     float(p.grady) * dy;
+  const float awloc = 1.0f + awm1;
+  const float aw2 = float(p.aw * p.aw);
+  const float db0 = px * px + py * py + aw2 * (awm1 * (2.0f + awm1));
+  const float ez = p.slice_eloss ? float(p.slice_eloss[slice]) : 0.0f;
+  float rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    gather<float>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
+  }
+  const float h = float(p.delz);
+  float k2gg = 0.0f, k2pp = 0.0f, k3gg, k3pp;
+  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  float stpz = 0.5f * h;
+  g += stpz * k2gg;
+  t += double(stpz * k2pp);
+  k3gg = k2gg;
+  k3pp = k2pp;
+  k2gg = 0.0f;
+  k2pp = 0.0f;
+  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  g += stpz * (k2gg - k3gg);
+  t += double(stpz * (k2pp - k3pp));
+  k3gg /= 6.0f;
+  k3pp /= 6.0f;
+  k2gg *= -0.5f;
+  k2pp *= -0.5f;
+  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  stpz = h;
+  g += stpz * k2gg;
+  t += double(stpz * k2pp);
+  k3gg -= k2gg;
+  k3pp -= k2pp;
+  k2gg *= 2.0f;
+  k2pp *= 2.0f;
+  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  g += stpz * (k3gg + k2gg / 6.0f);
+  t += double(stpz * (k3pp + k2pp / 6.0f));
+  dgamma = g;
+  theta = t;
+}
+
+__device__ __forceinline__ double gabs(double g, double) { return g; }
+__device__ __forceinline__ double gabs(float dg, double g0) { return g0 + double(dg); }
+__device__ __forceinline__ double gshift(double g, double g0) { return g - g0; }
+__device__ __forceinline__ double gshift(float dg, double) { return double(dg); }
+
+__device__ __forceinline__ void lds_add(double* a, double v) {
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void glob_add(double* a, double v) { unsafeAtomicAdd(a, v); }
+__device__ __forceinline__ void glob_add(float* a, float v) { unsafeAtomicAdd(a, v); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <class P, int NF, bool DEP, bool DIAG>
+__global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
+                                                            ParticleParams prm) {
+  using T2 = typename V2<P>::T;
+  extern __shared__ double2 win[];
+  __shared__ int s_bb[4];   // particle bbox min x, min y, max x, max y (lower-left cells)
+  __shared__ int s_fb[4];   // fallback (outside-window) touched-cell bbox
+  __shared__ double s_red[kWaves][kNSum + kNMinMax];
+
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t i0 = off[s], i1 = off[s + 1];
+  const int st = prm.stages;
+  const double g0 = prm.gamma0;
+
+  // ---- deposition set-up -------------------------------------------------
+  const int ng = prm.dep.ngrid;
+  int wx0 = 0, wy0 = 0, ww = 0, wh = 0;
+  P scl = P(0);
+  T2* src = nullptr;
+  if (DEP) {
+    src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
+    scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
+    const int4 bb = prm.dep.bbox[s];
+    if (bb.x <= bb.z && scl != P(0)) {
+      int x0 = max(0, bb.x - 2), y0 = max(0, bb.y - 2);
+      int x1 = min(ng, bb.z + 4), y1 = min(ng, bb.w + 4);
+      int w = x1 - x0, h = y1 - y0;
+      const int cap = prm.dep.window_cap;
+      if (w * h > cap) {
+        int side = 1;
+        while ((side + 1) * (side + 1) <= cap) ++side;
+        int nw = min(w, max(side, cap / h));
+        int nh = min(h, cap / nw);
+        x0 += (w - nw) / 2;
+        y0 += (h - nh) / 2;
+        w = nw;
+        h = nh;
+      }
+      wx0 = x0;
+      wy0 = y0;
+      ww = w;
+      wh = h;
+    }
+    // zero what the previous deposition left in this slice's source plane
+    const int4 old = prm.dep.rect[s];
+    const int ow = old.z - old.x, oh = old.w - old.y;
+    if (ow > 0 && oh > 0) {
+      for (int k = tid; k < ow * oh; k += kThreads) {
+        const int cy = old.y + k / ow, cx = old.x + k % ow;
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        src[cx + static_cast<int64_t>(cy) * ng] = z;
+      }
+    }
+    for (int k = tid; k < ww * wh; k += kThreads) win[k] = make_double2(0.0, 0.0);
+    if (tid == 0) {
+      s_bb[0] = s_bb[1] = 0x7fffffff;
+      s_bb[2] = s_bb[3] = -1;
+      s_fb[0] = s_fb[1] = 0x7fffffff;
+      s_fb[2] = s_fb[3] = -1;
+    }
+    __syncthreads();
+  }
+  int bminx = 0x7fffffff, bminy = 0x7fffffff, bmaxx = -1, bmaxy = -1;
+
+  // ---- diagnostics accumulators -------------------------------------------
+  double acc[kNSum];
+  double mm[kNMinMax];
+  const int nharm = DIAG ? prm.diag.nharm : 0;
+  if (DIAG) {
+#pragma unroll
+    for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
+    mm[0] = 1.0e5;  mm[1] = -1.0e5;  // x
+    mm[2] = 1.0e5;  mm[3] = -1.0e5;  // px
+    mm[4] = 1.0e5;  mm[5] = -1.0e5;  // y
+    mm[6] = 1.0e5;  mm[7] = -1.0e5;  // py
+    mm[8] = 1.0e7;  mm[9] = 1.0;     // gamma
+  }
+
+  MixedK mk;
+  if (sizeof(P) == 4) {
+    mk.ztemp1 = float(-2.0 / prm.push.xks);
+    mk.c0 = float(prm.push.c0);
+    mk.k1 = float(prm.push.k1);
+    mk.k2 = float(prm.push.k2);
+    mk.inv_b0 = float(prm.push.inv_b0);
+    mk.inv_g0 = float(1.0 / g0);
+  }
+
+  for (int64_t i = i0 + tid; i < i1; i += kThreads) {
+    P x = b.x[i], y = b.y[i], px = b.px[i], py = b.py[i], g = b.g[i];
+    double th = b.th[i];
+    if (st & ST_T1) transverse<P>(prm.t1, g0, g, x, y, px, py, true);
+    if (st & ST_PUSH) {
+      if constexpr (sizeof(P) == 8) {
+        double gg = g;
+        push64<NF>(prm.push, s, x, y, px, py, gg, th);
+        g = gg;
+      } else {
+        float gg = g;
+        push32<NF>(prm.push, mk, s, x, y, px, py, gg, th);
+        g = gg;
+      }
+    }
+    if (st & ST_R56) th += prm.r56 * gshift(g, g0);
+    if (st & ST_T2) transverse<P>(prm.t2, g0, g, x, y, px, py, false);
+    if (st & (ST_T1 | ST_T2)) {
+      b.x[i] = x;
+      b.y[i] = y;
+      b.px[i] = px;
+      b.py[i] = py;
+    }
+    if (st & ST_PUSH) b.g[i] = g;
+    if (st & (ST_PUSH | ST_R56)) b.th[i] = th;
+
+    if (DEP && scl != P(0)) {
+      const DepParams& d = prm.dep;
+      int ix, iy;
+      P wx, wy;
+      {
+        if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), ng, ix, iy, wx, wy)) {
+          const P dx = x - P(d.ax), dy = y - P(d.ay);
+          const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
+                         P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
+          P part;
+          P sn, cs;
+          if constexpr (sizeof(P) == 8) {
+            part = sqrt(faw2) * scl / g;
+            sincos(static_cast<double>(d.harm) * th, &sn, &cs);
+          } else {
+            part = sqrt(faw2) * scl / (float(g0) + g);
+            sincos_red(static_cast<double>(d.harm) * th, sn, cs);
+          }
+          const P cre = sn * part;
+          const P cim = cs * part;
+          const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
+          bminx = min(bminx, ix);
+          bminy = min(bminy, iy);
+          bmaxx = max(bmaxx, ix);
+          bmaxy = max(bmaxy, iy);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int cx = ix + (k & 1), cy = iy + (k >> 1);
+            const int lx = cx - wx0, ly = cy - wy0;
+            if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+              double* a = &win[lx + ly * ww].x;
+              lds_add(a, static_cast<double>(cre * w[k]));
+              lds_add(a + 1, static_cast<double>(cim * w[k]));
+            } else {
+              P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
+              glob_add(a, cre * w[k]);
+              glob_add(a + 1, cim * w[k]);
+              atomicMin(&s_fb[0], cx);
+              atomicMin(&s_fb[1], cy);
+              atomicMax(&s_fb[2], cx);
+              atomicMax(&s_fb[3], cy);
+            }
+          }
+        }
+      }
+    }
+
+    if (DIAG) {
+      const double xv = x, yv = y, pxv = px, pyv = py;
+      const double dg = gshift(g, g0);
+      const double ga = gabs(g, g0);
+      acc[0] += dg;
+      acc[1] += dg * dg;
+      acc[2] += xv;
+      acc[3] += xv * xv;
+      acc[4] += yv;
+      acc[5] += yv * yv;
+      acc[6] += pxv;
+      acc[7] += pxv * pxv;
+      acc[8] += pyv;
+      acc[9] += pyv * pyv;
+      acc[10] += xv * pxv;
+      acc[11] += yv * pyv;
+      acc[12] += 1.0;
+      mm[0] = fmin(mm[0], xv);
+      mm[1] = fmax(mm[1], xv);
+      mm[2] = fmin(mm[2], pxv);
+      mm[3] = fmax(mm[3], pxv);
+      mm[4] = fmin(mm[4], yv);
+      mm[5] = fmax(mm[5], yv);
+      mm[6] = fmin(mm[6], pyv);
+      mm[7] = fmax(mm[7], pyv);
+      mm[8] = fmin(mm[8], ga);
+      mm[9] = fmax(mm[9], ga);
+#pragma unroll
+      for (int h = 0; h < kMaxHarm; ++h) {
+        if (h < nharm) {
+          if constexpr (sizeof(P) == 8) {
+            double sn, cs;
+            sincos(static_cast<double>(h + 1) * th, &sn, &cs);
+            acc[13 + 2 * h] += cs;
+            acc[14 + 2 * h] += sn;
+          } else {
+            float sn, cs;
+            sincos_red(static_cast<double>(h + 1) * th, sn, cs);
+            acc[13 + 2 * h] += cs;
+            acc[14 + 2 * h] += sn;
+          }
+        }
+      }
+    }
+  }
+
+  if (DEP) {
+    atomicMin(&s_bb[0], bminx);
+    atomicMin(&s_bb[1], bminy);
+    atomicMax(&s_bb[2], bmaxx);
+    atomicMax(&s_bb[3], bmaxy);
+    __syncthreads();
+    for (int k = tid; k < ww * wh; k += kThreads) {
+      const int cy = wy0 + k / ww, cx = wx0 + k % ww;
+      const double2 v = win[k];
+      T2 o;
+      o.x = static_cast<P>(v.x);
+      o.y = static_cast<P>(v.y);
+      src[cx + static_cast<int64_t>(cy) * ng] = o;
+    }
+    if (tid == 0) {
+      int4 r;
+      if (ww > 0 && wh > 0) {
+        r = make_int4(wx0, wy0, wx0 + ww, wy0 + wh);
+      } else {
+        r = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
+      }
+      if (s_fb[0] <= s_fb[2]) {
+        r.x = min(r.x, s_fb[0]);
+        r.y = min(r.y, s_fb[1]);
+        r.z = max(r.z, s_fb[2] + 1);
+        r.w = max(r.w, s_fb[3] + 1);
+      }
+      if (r.x > r.z) r = make_int4(0, 0, 0, 0);
+      prm.dep.rect[s] = r;
+      prm.dep.bbox[s] = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
+    }
+  }
+
+  if (DIAG) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nsum = 13 + 2 * nharm;
+#pragma unroll
+    for (int k = 0; k < kNSum; ++k) {
+      if (k < nsum) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) s_red[wave][k] = v;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kNMinMax; ++k) {
+      const double v = (k & 1) ? wave_max(mm[k]) : wave_min(mm[k]);
+      if (lane == 0) s_red[wave][kNSum + k] = v;
+    }
+    __syncthreads();
+    double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
+    // raw layout: 13 sums (G4H_B_G1..G4H_B_COUNT), 10 min/max, then bunching
+    if (tid < nsum) {
+      double v = 0.0;
+      for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+      const int dst = tid < 13 ? tid : G4H_BEAM_STATS + (tid - 13);
+      out[dst] = v;
+    } else if (tid >= 64 && tid < 64 + kNMinMax) {
+      const int k = tid - 64;
+      double v = s_red[0][kNSum + k];
+      for (int w = 1; w < kWaves; ++w)
+        v = (k & 1) ? fmax(v, s_red[w][kNSum + k]) : fmin(v, s_red[w][kNSum + k]);
+      // mm order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
+      out[G4H_B_XMIN + k] = v;
+    }
+  }
+}
+
+template <class P>
+__global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg, int ng,
+                                  int32_t* cell) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  int ix, iy;
+  P wx, wy;
+  cell[i] = cell_of<P>(b.x[i], b.y[i], P(gm), P(dg), ng, ix, iy, wx, wy) ? ix + iy * ng : -1;
+}
+
+template <class P, int NF, bool DEP, bool DIAG>
+void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
+  const size_t lds = DEP ? static_cast<size_t>(p.dep.window_cap) * sizeof(double2) : 0;
+  ctx->launch(G4H_K_PARTICLE, [&] {
+    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, DIAG>), dim3(b->nslice), dim3(kThreads), lds,
+                       ctx->stream, bp, b->off.get(), p);
+  });
+}
+
+template <class P>
+void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
+  const bool dep = (p.stages & ST_DEP) != 0;
+  const bool diag = (p.stages & ST_DIAG) != 0;
+  const bool single = !(p.stages & ST_PUSH) || p.push.nfield == 1;
+  if (single) {
+    if (dep && diag) return launch_t<P, 1, true, true>(ctx, b, p, bp);
+    if (dep) return launch_t<P, 1, true, false>(ctx, b, p, bp);
+    if (diag) return launch_t<P, 1, false, true>(ctx, b, p, bp);
+    return launch_t<P, 1, false, false>(ctx, b, p, bp);
+  }
+  if (dep && diag) return launch_t<P, 0, true, true>(ctx, b, p, bp);
+  if (dep) return launch_t<P, 0, true, false>(ctx, b, p, bp);
+  if (diag) return launch_t<P, 0, false, true>(ctx, b, p, bp);
+  return launch_t<P, 0, false, false>(ctx, b, p, bp);
+}
+
+}  // namespace
+
+void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell) {
+  if (b->np <= 0) return;
+  const unsigned blocks = static_cast<unsigned>((b->np + 255) / 256);
+  ctx->launch(G4H_K_OTHER, [&] {
+    if (b->prec == G4H_FP64) {
+      BeamPtr<double> bp{b->xd.get(), b->yd.get(), b->pxd.get(), b->pyd.get(), b->gd.get(),
+                         b->theta.get()};
+      hipLaunchKernelGGL(cell_index_kernel<double>, dim3(blocks), dim3(256), 0, ctx->stream, bp,
+                         b->np, f->gridmax, f->dgrid, f->ngrid, d_cell);
+    } else {
+      BeamPtr<float> bp{b->xf.get(), b->yf.get(), b->pxf.get(), b->pyf.get(), b->gf.get(),
+                        b->theta.get()};
+      hipLaunchKernelGGL(cell_index_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream, bp,
+                         b->np, f->gridmax, f->dgrid, f->ngrid, d_cell);
+    }
+  });
+}
+
+void launch_particle(Context* ctx, Beam* b, const ParticleParams& p) {
+  if (b->nslice <= 0) return;
+  require(p.dep.window_cap * sizeof(double2) <= 96 * 1024 || !(p.stages & ST_DEP),
+          "deposition window exceeds the LDS budget");
+  require(!(p.stages & ST_DIAG) || (p.diag.nharm >= 1 && p.diag.nharm <= kMaxHarm),
+          "nharm must be in [1, 8]");
+  if (b->prec == G4H_FP64) {
+    BeamPtr<double> bp{b->xd.get(), b->yd.get(), b->pxd.get(), b->pyd.get(), b->gd.get(),
+                       b->theta.get()};
+    dispatch<double>(ctx, b, p, bp);
+  } else {
+    BeamPtr<float> bp{b->xf.get(), b->yf.get(), b->pxf.get(), b->pyf.get(), b->gf.get(),
+                      b->theta.get()};
+    dispatch<float>(ctx, b, p, bp);
+  }
+}
+
+}  // namespace g4

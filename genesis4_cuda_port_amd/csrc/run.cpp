@@ -1,0 +1,308 @@
+// run.cpp - the fused step loop (Gencore::run, src/Core/Gencore.cpp:161-254)
+// and the host finalisation of the diagnostics (DiagnosticCUDA.cpp:509-790).
+//
+// Per lattice step the run enqueues, on the context's stream and without any
+// host synchronisation:
+//   1. one particle pass: half transverse (+chicane), gather+RK4, R56, half
+//      transverse (+corrector), deposition into the source planes and, on
+//      output steps, the per-slice beam sums      (Beam::track + source part of
+//      FieldSolver*::advance + DiagnosticCUDA beam kernel);
+//   2. the field solver passes (FFT rows+columns or ADI x+y sweeps), the last
+//      one reducing the per-plane field sums on output steps;
+//   3. the slippage ring shift (host integer update + zero fill of the
+//      incoming plane);
+//   4. on output steps, the map of per-plane field sums to logical slices.
+// Diagnostics stay in device history buffers until g4h_run_download_diag.
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace g4 {
+int slippage_host(Field* f, double slippage, double sample, int timerun, int periodic);
+}
+
+using namespace g4;
+
+struct g4h_run {
+  Context* ctx = nullptr;
+  Beam* b = nullptr;
+  Field* f = nullptr;
+  std::vector<g4h_step> steps;
+  g4h_run_config cfg{};
+  int done = 0;
+  int nrec = 0;
+  int cap = 0;
+  DevBuf<double> beam_hist, field_hist;
+
+  size_t beam_rec() const { return static_cast<size_t>(b->nslice) * G4H_BEAM_RAW(cfg.nharm); }
+  size_t field_rec() const { return static_cast<size_t>(f->nslice) * G4H_FIELD_RAW; }
+
+  void record_initial() {
+    if (!cfg.record_diag) return;
+    ParticleParams p{};
+    p.gamma0 = b->gamma_ref;
+    p.stages = ST_DIAG;
+    p.diag.raw = beam_hist.get() + nrec * beam_rec();
+    p.diag.nharm = cfg.nharm;
+    launch_particle(ctx, b, p);
+    field_plane_stats(f);
+    field_map_stats(f, field_hist.get() + nrec * field_rec());
+    ++nrec;
+  }
+
+  void step(const g4h_step& st) {
+    const double delz = st.delz;
+    const double g0 = b->gamma_ref;
+    const bool out = cfg.record_diag && st.outstep && nrec < cap;
+    field_prepare_solver(f, delz);
+    ParticleParams p{};
+    p.gamma0 = g0;
+    p.stages = ST_T1 | ST_PUSH | ST_T2;
+    p.t1 = make_trans(st, g0, delz * 0.5, false);
+    p.t2 = make_trans(st, g0, delz * 0.5, true);
+    Field* fl[1] = {f};
+    make_push(p.push, st, g0, delz, fl, 1, nullptr);
+    p.r56 = make_r56(st, g0, cfg.lambda0);
+    if (p.r56 != 0.0) p.stages |= ST_R56;
+    const bool src_on = make_dep(p.dep, f, b, st, delz) && b->np > 0;
+    if (src_on) p.stages |= ST_DEP;
+    if (out) {
+      p.stages |= ST_DIAG;
+      p.diag.raw = beam_hist.get() + nrec * beam_rec();
+      p.diag.nharm = cfg.nharm;
+    }
+    launch_particle(ctx, b, p);
+    field_solve(f, src_on, out);
+    slippage_host(f, st.slippage, cfg.sample, cfg.timerun, cfg.periodic);
+    if (out) {
+      field_map_stats(f, field_hist.get() + nrec * field_rec());
+      ++nrec;
+    }
+  }
+};
+
+#define G4_API_BEGIN try {
+#define G4_API_END                     \
+  return G4H_OK;                       \
+  }                                    \
+  catch (const g4::Error& e) {         \
+    g4::set_last_error(e.what());      \
+    return e.code;                     \
+  }                                    \
+  catch (const std::exception& e) {    \
+    g4::set_last_error(e.what());      \
+    return G4H_ERR_DEVICE;             \
+  }
+
+extern "C" {
+
+g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,
+                          const g4h_step* steps, int nsteps, const g4h_run_config* cfg,
+                          g4h_run** out) {
+  G4_API_BEGIN
+  require(ctx && beam && field && cfg && out, "null argument");
+  *out = nullptr;
+  require(nsteps >= 0 && (nsteps == 0 || steps), "bad step table");
+  require(beam->uploaded, "beam state was never uploaded");
+  require(beam->prec == field->prec, "beam and field precision differ");
+  require(beam->nslice == field->nslice, "beam and field slice counts differ");
+  require(cfg->nharm >= 1 && cfg->nharm <= 8, "nharm must be in [1, 8]");
+  require(cfg->lambda0 > 0, "lambda0 must be positive");
+  require(!cfg->timerun || cfg->sample > 0, "sample must be positive for time-dependent runs");
+  for (int i = 0; i < nsteps; ++i) require(steps[i].delz > 0, "step length must be positive");
+  ctx->activate();
+  auto* r = new g4h_run();
+  try {
+    r->ctx = ctx;
+    r->b = beam;
+    r->f = field;
+    r->steps.assign(steps, steps + nsteps);
+    r->cfg = *cfg;
+    int nout = 0;
+    for (int i = 0; i < nsteps; ++i) nout += steps[i].outstep ? 1 : 0;
+    r->cap = cfg->record_diag ? 1 + nout : 0;
+    if (r->cap > 0) {
+      r->beam_hist.alloc(ctx, r->cap * r->beam_rec());
+      r->field_hist.alloc(ctx, r->cap * r->field_rec());
+    }
+    r->record_initial();
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  *out = r;
+  G4_API_END
+}
+
+void g4h_run_destroy(g4h_run* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->ctx->device);
+  (void)hipStreamSynchronize(r->ctx->stream);
+  delete r;
+}
+
+g4h_status g4h_run_advance(g4h_run* r, int nsteps, int* done) {
+  G4_API_BEGIN
+  require(r, "run is null");
+  require(nsteps >= 0, "nsteps must be >= 0");
+  r->ctx->activate();
+  const int end = std::min<int>(static_cast<int>(r->steps.size()), r->done + nsteps);
+  for (; r->done < end; ++r->done) r->step(r->steps[r->done]);
+  if (done) *done = r->done;
+  G4_API_END
+}
+
+g4h_status g4h_run_records(const g4h_run* r, int* n) {
+  G4_API_BEGIN
+  require(r && n, "null argument");
+  *n = r->nrec;
+  G4_API_END
+}
+
+g4h_status g4h_run_download_diag(g4h_run* r, double* beam_raw, double* field_raw) {
+  G4_API_BEGIN
+  require(r, "run is null");
+  r->ctx->activate();
+  G4_HIP(hipStreamSynchronize(r->ctx->stream));
+  if (r->nrec > 0 && beam_raw)
+    G4_HIP(hipMemcpy(beam_raw, r->beam_hist.get(), sizeof(double) * r->nrec * r->beam_rec(),
+                     hipMemcpyDeviceToHost));
+  if (r->nrec > 0 && field_raw)
+    G4_HIP(hipMemcpy(field_raw, r->field_hist.get(), sizeof(double) * r->nrec * r->field_rec(),
+                     hipMemcpyDeviceToHost));
+  G4_API_END
+}
+
+// DiagnosticCUDA::calcBeam host part (DiagnosticCUDA.cpp:509-628), all output
+// filters on, one rank.  Energy sums are relative to gamma_ref.
+g4h_status g4h_diag_finalize_beam(int nslice, int nharm, const double* raw, const double* current,
+                                  double gamma_ref, double* q, double* global) {
+  G4_API_BEGIN
+  require(raw && q && global, "null argument");
+  require(nslice > 0 && nharm >= 1, "bad sizes");
+  const int rs = G4H_BEAM_RAW(nharm), qs = G4H_BEAM_QN(nharm);
+  double gc = 0, gg1 = 0, gg2 = 0, gx1 = 0, gx2 = 0, gy1 = 0, gy2 = 0;
+  for (int is = 0; is < nslice; ++is) {
+    const double* s = raw + static_cast<size_t>(is) * rs;
+    double* o = q + static_cast<size_t>(is) * qs;
+    const double cnt = s[G4H_B_COUNT];
+    const double nrm = cnt > 0.0 ? 1.0 / cnt : 1.0;
+    const double x1 = s[G4H_B_X1] * nrm, x2 = s[G4H_B_X2] * nrm;
+    const double y1 = s[G4H_B_Y1] * nrm, y2 = s[G4H_B_Y2] * nrm;
+    const double px1 = s[G4H_B_PX1] * nrm, px2 = s[G4H_B_PX2] * nrm;
+    const double py1 = s[G4H_B_PY1] * nrm, py2 = s[G4H_B_PY2] * nrm;
+    const double d1 = s[G4H_B_G1] * nrm, d2 = s[G4H_B_G2] * nrm;
+    const double g1 = cnt > 0.0 ? gamma_ref + d1 : 0.0;
+    const double xpx = s[G4H_B_XPX] * nrm, ypy = s[G4H_B_YPY] * nrm;
+    const double vx = x2 - x1 * x1, vpx = px2 - px1 * px1, cx = xpx - x1 * px1;
+    const double vy = y2 - y1 * y1, vpy = py2 - py1 * py1, cy = ypy - y1 * py1;
+    o[G4H_Q_ENERGY] = g1;
+    o[G4H_Q_ENERGYSPREAD] = std::sqrt(std::fabs(d2 - d1 * d1));
+    o[G4H_Q_XPOSITION] = x1;
+    o[G4H_Q_XSIZE] = std::sqrt(std::fabs(vx));
+    o[G4H_Q_YPOSITION] = y1;
+    o[G4H_Q_YSIZE] = std::sqrt(std::fabs(vy));
+    o[G4H_Q_PXPOSITION] = px1;
+    o[G4H_Q_PYPOSITION] = py1;
+    const double cur = current ? current[is] : 0.0;
+    o[G4H_Q_CURRENT] = cur;
+    const double ex = std::sqrt(std::fabs(vx * vpx - cx * cx));
+    const double ey = std::sqrt(std::fabs(vy * vpy - cy * cy));
+    o[G4H_Q_EMITX] = ex;
+    o[G4H_Q_EMITY] = ey;
+    o[G4H_Q_BETAX] = ex > 0.0 ? vx / ex * g1 : 0.0;
+    o[G4H_Q_BETAY] = ey > 0.0 ? vy / ey * g1 : 0.0;
+    o[G4H_Q_ALPHAX] = ex > 0.0 ? -cx / ex : 0.0;
+    o[G4H_Q_ALPHAY] = ey > 0.0 ? -cy / ey : 0.0;
+    for (int k = 0; k < 10; ++k) o[G4H_Q_XMIN + k] = s[G4H_B_XMIN + k];
+    for (int ih = 0; ih < nharm; ++ih) {
+      const double bre = s[G4H_BEAM_STATS + 2 * ih] * nrm;
+      const double bim = s[G4H_BEAM_STATS + 2 * ih + 1] * nrm;
+      o[G4H_BEAM_Q + 2 * ih] = std::hypot(bre, bim);
+      o[G4H_BEAM_Q + 2 * ih + 1] = std::atan2(bim, bre);
+    }
+    gc += cur;
+    gg1 += cur * d1;
+    gg2 += cur * d2;
+    gx1 += cur * x1;
+    gx2 += cur * x2;
+    gy1 += cur * y1;
+    gy2 += cur * y2;
+  }
+  const double nrm = gc > 0.0 ? 1.0 / gc : 1.0;
+  gg1 *= nrm;
+  gg2 *= nrm;
+  gx1 *= nrm;
+  gx2 *= nrm;
+  gy1 *= nrm;
+  gy2 *= nrm;
+  global[G4H_BG_ENERGY] = gamma_ref + gg1;
+  global[G4H_BG_ENERGYSPREAD] = std::sqrt(std::fabs(gg2 - gg1 * gg1));
+  global[G4H_BG_XPOSITION] = gx1;
+  global[G4H_BG_XSIZE] = std::sqrt(std::fabs(gx2 - gx1 * gx1));
+  global[G4H_BG_YPOSITION] = gy1;
+  global[G4H_BG_YSIZE] = std::sqrt(std::fabs(gy2 - gy1 * gy1));
+  G4_API_END
+}
+
+// DiagnosticCUDA::calcField host part (DiagnosticCUDA.cpp:685-790), one rank.
+g4h_status g4h_diag_finalize_field(int nslice, double dgrid, double xlambda, double slicelength,
+                                   const double* raw, double* q, double* global) {
+  G4_API_BEGIN
+  require(raw && q && global, "null argument");
+  require(nslice > 0 && dgrid > 0 && xlambda > 0, "bad sizes");
+  const double ks = 4.0 * std::asin(1.0) / xlambda;
+  const double scl = dgrid * G4H_EEV / ks;
+  double gp = 0, gx1 = 0, gx2 = 0, gy1 = 0, gy2 = 0, gff = 0, gin = 0;
+  for (int is = 0; is < nslice; ++is) {
+    const double* s = raw + static_cast<size_t>(is) * G4H_FIELD_RAW;
+    double* o = q + static_cast<size_t>(is) * G4H_FIELD_Q;
+    const double pr = s[G4H_F_POWER];
+    double x1 = s[G4H_F_X1], x2 = s[G4H_F_X2], y1 = s[G4H_F_Y1], y2 = s[G4H_F_Y2];
+    gp += pr;
+    gx1 += x1;
+    gx2 += x2;
+    gy1 += y1;
+    gy2 += y2;
+    if (pr > 0.0) {
+      x1 /= pr;
+      x2 /= pr;
+      y1 /= pr;
+      y2 /= pr;
+    }
+    x2 = std::sqrt(std::fabs(x2 - x1 * x1));
+    y2 = std::sqrt(std::fabs(y2 - y1 * y1));
+    const double cre = s[G4H_F_CENTER_RE], cim = s[G4H_F_CENTER_IM];
+    double inten = cre * cre + cim * cim;
+    double ff = s[G4H_F_FF_RE] * s[G4H_F_FF_RE] + s[G4H_F_FF_IM] * s[G4H_F_FF_IM];
+    ff *= dgrid * dgrid;
+    gff += ff;
+    gin += inten;
+    o[G4H_FQ_POWER] = pr * scl * scl / G4H_VACIMP;
+    o[G4H_FQ_XPOSITION] = x1 * dgrid;
+    o[G4H_FQ_XSIZE] = x2 * dgrid;
+    o[G4H_FQ_YPOSITION] = y1 * dgrid;
+    o[G4H_FQ_YSIZE] = y2 * dgrid;
+    o[G4H_FQ_INTENSITY_NF] = inten * G4H_EEV * G4H_EEV / ks / ks / G4H_VACIMP;
+    o[G4H_FQ_PHASE_NF] = std::atan2(cim, cre);
+    o[G4H_FQ_INTENSITY_FF] = ff;
+    o[G4H_FQ_PHASE_FF] = std::atan2(s[G4H_F_FF_IM], s[G4H_F_FF_RE]);
+  }
+  const double nrm = gp > 0.0 ? 1.0 / gp : 1.0;
+  gx1 *= nrm;
+  gx2 *= nrm;
+  gy1 *= nrm;
+  gy2 *= nrm;
+  global[G4H_FG_ENERGY] = gp * scl * scl / G4H_VACIMP * slicelength / G4H_CLIGHT;
+  global[G4H_FG_XPOSITION] = gx1 * dgrid;
+  global[G4H_FG_XSIZE] = std::sqrt(std::fabs(gx2 - gx1 * gx1)) * dgrid;
+  global[G4H_FG_YPOSITION] = gy1 * dgrid;
+  global[G4H_FG_YSIZE] = std::sqrt(std::fabs(gy2 - gy1 * gy1)) * dgrid;
+  const double avg = 1.0 / static_cast<double>(nslice);
+  global[G4H_FG_INTENSITY_NF] = gin * avg * G4H_EEV * G4H_EEV / ks / ks / G4H_VACIMP;
+  global[G4H_FG_INTENSITY_FF] = gff * avg;
+  G4_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+/*
+ * g4hip.h - C ABI of the MI355X-native GENESIS 1.3 v4 time-dependent step.
+ *
+ * This is the drop-in boundary.  Each entry point replaces one call site of
+ * the reference CUDA overlay (mulingLHY/Genesis4_CUDA_port, paths relative to
+ * its root) and is cited next to its declaration.  All arguments are plain
+ * pointers, sizes and PODs; no exception crosses this boundary: every call
+ * returns a g4h_status and leaves a message in g4h_last_error().
+ *
+ * Threading: a g4h_context owns one HIP stream.  A context and every object
+ * created from it are used from one host thread at a time.  Calls enqueue
+ * work on the context's stream and return without synchronising unless they
+ * hand data back to the host.
+ */
+#ifndef G4HIP_H_
+#define G4HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "g4hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct g4h_context g4h_context;
+typedef struct g4h_beam g4h_beam;
+typedef struct g4h_field g4h_field;
+typedef struct g4h_run g4h_run;
+
+/* ------------------------------------------------------------------------ */
+/* Library / context (replaces genesis4_cuda::initialize/check/stream/        */
+/* synchronize, include/Genesis4CudaRuntime.h:13-50)                          */
+/* ------------------------------------------------------------------------ */
+
+/* ABI version; bumps on any incompatible change. */
+int g4h_abi_version(void);
+
+/* Number of visible GPUs (0 when none); never fails. */
+int g4h_device_count(void);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char* g4h_last_error(void);
+
+/* Create a context on `device` with its own stream (default priority). */
+g4h_status g4h_context_create(int device, g4h_context** out);
+void g4h_context_destroy(g4h_context* ctx);
+/* Wait for all work enqueued on the context's stream. */
+g4h_status g4h_context_synchronize(g4h_context* ctx);
+/* Device bytes currently held by objects of this context. */
+g4h_status g4h_context_memory(const g4h_context* ctx, uint64_t* bytes_in_use);
+
+/* ------------------------------------------------------------------------ */
+/* Beam: device-resident SoA (replaces Genesis4BeamSoA, include/              */
+/* Genesis4BeamSoA.h:7-21, and Beam::init_beamSoA/pack_beam_to_soa/          */
+/* unpack_soa_to_beam, src/Core/Beam.cpp:342-525)                             */
+/* ------------------------------------------------------------------------ */
+
+/* slice_offsets[nslice+1]: particles of slice s are [off[s], off[s+1]).
+ * gamma_ref is the reference energy (Undulator::getGammaRef); in G4H_MIXED
+ * the energy is stored as FP32 gamma - gamma_ref.  current[nslice] [A] is the
+ * slice current used for the source scale (Beam::current). */
+g4h_status g4h_beam_create(g4h_context* ctx, g4h_precision precision, int nslice,
+                           const int32_t* slice_offsets, const double* current,
+                           double gamma_ref, g4h_beam** out);
+void g4h_beam_destroy(g4h_beam* beam);
+/* Host FP64 arrays of total_particles entries each (Beam::pack_beam_to_soa). */
+g4h_status g4h_beam_upload(g4h_beam* beam, const double* x, const double* y,
+                           const double* px, const double* py, const double* gamma,
+                           const double* theta);
+/* Blocking copy back to host FP64 arrays (Beam::unpack_soa_to_beam). */
+g4h_status g4h_beam_download(g4h_beam* beam, double* x, double* y, double* px,
+                             double* py, double* gamma, double* theta);
+g4h_status g4h_beam_set_current(g4h_beam* beam, const double* current);
+g4h_status g4h_beam_info(const g4h_beam* beam, int* nslice, int64_t* total_particles,
+                         int* precision);
+
+/* ------------------------------------------------------------------------ */
+/* Field: device-resident complex planes in a ring of slices (replaces        */
+/* Genesis4FieldSoA, include/Genesis4FieldSoA.h:7-17, and Field::init_fieldSoA/*/
+/* pack_field_to_soa/unpack_soa_to_field, src/Core/Field.cpp:582-674)         */
+/* ------------------------------------------------------------------------ */
+
+/* gridmax is the half width (Field::gridmax); dgrid = 2*gridmax/(ngrid-1)
+ * (Field.cpp:60-61).  xlambda0 is the fundamental wavelength; the field's
+ * wavelength is xlambda0/harm (Field.cpp:57). */
+g4h_status g4h_field_create(g4h_context* ctx, g4h_precision precision, int nslice,
+                            int ngrid, double gridmax, double xlambda0, double slicelength,
+                            int harm, g4h_field** out);
+void g4h_field_destroy(g4h_field* field);
+/* Storage-order planes (plane p holds ring slot p), cell ix + iy*ngrid;
+ * `first` is the ring offset (Field::first). */
+g4h_status g4h_field_upload(g4h_field* field, const double* re, const double* im,
+                            int first);
+g4h_status g4h_field_download(g4h_field* field, double* re, double* im, int* first);
+/* Ring state used by the slippage (Field::first, Field::accuslip). */
+g4h_status g4h_field_get_ring(const g4h_field* field, int* first, double* accuslip);
+g4h_status g4h_field_set_ring(g4h_field* field, int first, double accuslip);
+/* Field::disable / isEnabled (include/Field.h:117-130). */
+g4h_status g4h_field_set_enabled(g4h_field* field, int enabled);
+
+/* Field::initSolver (src/Core/Field.cpp:87-110): solver kind, FFT source
+ * filter (FieldSolverFFTCUDA::initSourceFilter, FieldSolverFFTCUDA.cpp:
+ * 213-221).  The ADI low-memory switch is accepted for deck compatibility;
+ * this build never allocates a dense source grid, so both settings run the
+ * same kernels. */
+g4h_status g4h_field_init_solver(g4h_field* field, g4h_solver_kind kind, int source_filter,
+                                 double xcut, double ycut, double sigmoid,
+                                 int low_memory_adisolver);
+
+/* ------------------------------------------------------------------------ */
+/* Step operators (the hot path)                                              */
+/* ------------------------------------------------------------------------ */
+
+/* TrackBeamCUDA::track (src/Core/TrackBeamCUDA.cpp:156-204) including
+ * applyChicane on the first half step and applyCorrector on the last
+ * (:181-185, :206-255).  delz is the half step. */
+g4h_status g4h_track_transverse(g4h_context* ctx, g4h_beam* beam, const g4h_step* und,
+                                double delz, int last_step);
+
+/* TrackBeamCUDA::applyR56 (src/Core/TrackBeamCUDA.cpp:257-278); lambda0 is
+ * Beam::reflength (Beam.cpp:130). */
+g4h_status g4h_apply_r56(g4h_context* ctx, g4h_beam* beam, const g4h_step* und,
+                         double lambda0);
+
+/* BeamSolverCUDA::advance (src/Core/BeamSolverCUDA.cpp:449-683): bilinear
+ * gather of up to 8 fields and the RK4 (gamma, theta) push.  slice_eloss
+ * (host, nslice values, may be NULL) is -longESC/511000 per slice (:555). */
+g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* beam, g4h_field* const* fields,
+                            int nfield, const g4h_step* und, double delz,
+                            const double* slice_eloss);
+
+/* Field::track = solver->init + solver->advance (src/Core/Field.cpp:204-212;
+ * FieldSolverADICUDA::advance :638-818, FieldSolverFFTCUDA::advance
+ * :325-483): source deposition from the beam, then one ADI or FFT step. */
+g4h_status g4h_field_track(g4h_context* ctx, g4h_field* field, g4h_beam* beam,
+                           const g4h_step* und, double delz);
+
+/* ControlCUDA::applySlippage (src/Core/ControlCUDA.cpp:148-252), one rank.
+ * sample = slicelength/reflength (Gencore.cpp:72).  *shifts (may be NULL)
+ * receives the number of ring shifts performed. */
+g4h_status g4h_apply_slippage(g4h_context* ctx, g4h_field* field, double slippage,
+                              double sample, int timerun, int periodic, int* shifts);
+
+/* Particle-to-cell index of every particle on `field`'s grid: ix + iy*ngrid
+ * of the lower-left cell, or -1 off the grid, computed by the same device
+ * function the gather and the deposition use (BeamSolverCUDA.cpp:99-116,
+ * FieldSolverADICUDA.cpp:126-143).  cell[total_particles] host; blocking. */
+g4h_status g4h_beam_cell_index(g4h_context* ctx, g4h_beam* beam, const g4h_field* field,
+                               int32_t* cell);
+
+/* DiagnosticCUDA::calc raw sums (src/Core/DiagnosticCUDA.cpp:188-363,
+ * 414-447, 633-659).  beam_raw: nslice*G4H_BEAM_RAW(nharm) doubles, energy
+ * sums relative to the beam's gamma_ref; field_raw: nslice*G4H_FIELD_RAW in
+ * logical slice order.  Blocking. */
+g4h_status g4h_diag_beam_raw(g4h_context* ctx, g4h_beam* beam, int nharm, double* beam_raw);
+g4h_status g4h_diag_field_raw(g4h_context* ctx, g4h_field* field, double* field_raw);
+
+/* Host finalisation (DiagnosticCUDA.cpp:509-628 and 685-790): raw sums to
+ * the named per-slice quantities (layouts in g4hip_types.h) and globals. */
+g4h_status g4h_diag_finalize_beam(int nslice, int nharm, const double* beam_raw,
+                                  const double* current, double gamma_ref, double* q,
+                                  double* global);
+g4h_status g4h_diag_finalize_field(int nslice, double dgrid, double xlambda,
+                                   double slicelength, const double* field_raw, double* q,
+                                   double* global);
+
+/* ------------------------------------------------------------------------ */
+/* Fused run: the Gencore::run loop (src/Core/Gencore.cpp:161-254) with the  */
+/* beam push, deposition and beam diagnostics fused into one particle pass   */
+/* and the field diagnostics fused into the last field-solver pass.          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct g4h_run_config {
+  double lambda0;      /* Beam::reflength [m]                              */
+  double sample;       /* slicelength/reflength                            */
+  int32_t timerun;     /* time-dependent (slippage on)                     */
+  int32_t periodic;    /* periodic time window                             */
+  int32_t nharm;       /* bunching harmonics recorded (&track bunchharm)   */
+  int32_t record_diag; /* 1: record diagnostics on outsteps                */
+} g4h_run_config;
+
+/* steps[nsteps] is copied.  The run records diagnostics before the first
+ * step and after every step with outstep != 0 into device history buffers.
+ * The beam's and field's precision must match. */
+g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,
+                          const g4h_step* steps, int nsteps, const g4h_run_config* cfg,
+                          g4h_run** out);
+void g4h_run_destroy(g4h_run* run);
+/* Execute up to `nsteps` further lattice steps (asynchronous). *done
+ * (may be NULL) receives the number of steps executed so far. */
+g4h_status g4h_run_advance(g4h_run* run, int nsteps, int* done);
+/* Number of diagnostic records written so far. */
+g4h_status g4h_run_records(const g4h_run* run, int* nrecords);
+/* Blocking copy of records [0, nrecords) (layouts as g4h_diag_*_raw). */
+g4h_status g4h_run_download_diag(g4h_run* run, double* beam_raw, double* field_raw);
+
+/* Per-kernel timing with HIP events on the context's stream.  While on,
+ * every launch is bracketed by events; g4h_profile_read returns for kernel
+ * class k (g4h_kernel_class) the number of launches and total milliseconds. */
+typedef enum g4h_kernel_class {
+  G4H_K_PARTICLE = 0,  /* fused particle pass (push, deposit, beam diag)   */
+  G4H_K_FFT_ROWS = 1,
+  G4H_K_FFT_COLS = 2,
+  G4H_K_ADI_X = 3,
+  G4H_K_ADI_Y = 4,
+  G4H_K_OTHER = 5,
+  G4H_K_COUNT = 6
+} g4h_kernel_class;
+g4h_status g4h_profile_enable(g4h_context* ctx, int on);
+g4h_status g4h_profile_read(g4h_context* ctx, int kernel_class, int64_t* launches,
+                            double* total_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Input front-end (host only, no GPU): the pieces of the upstream loaders   */
+/* the configurations need.  None of them is part of the reference; they are */
+/* restated from the published methods named in each comment.                */
+/* ------------------------------------------------------------------------ */
+
+/* Undulator coupling fc(h) = aw*[JJ]_h (planar) or aw (helical, h=1). */
+double g4h_undulator_fc(double aw, int helical, int harm);
+
+/* Unscrambled Sobol points (Joe-Kuo direction numbers), dims <= 8, points
+ * index0 .. index0+n-1, written row-major out[n*dims], values in [0,1). */
+g4h_status g4h_sobol(int dims, int64_t index0, int64_t n, double* out);
+
+/* Inverse standard normal CDF (Wichura AS241). */
+double g4h_inverse_normal(double p);
+
+/* Quiet-start beam loader for one run.  Fills x,y,px,py,gamma,theta for
+ * nslice slices of npart particles (npart a multiple of nbins): beamlets of
+ * nbins particles share (x,px,y,py,gamma) from a seeded (digitally shifted)
+ * Sobol sequence mapped through the inverse normal CDF, and their phases are
+ * mirrored at 2*pi*k/nbins.  gamma gets gamma0*(1+espread*g) plus
+ * chirp_rel*gamma0*(s/(nslice-1)-0.5).  If shotnoise != 0, phases get the
+ * Fawley random perturbation for the electron count of each beamlet
+ * (current[s]*slicelength/G4H_CE/nbeamlets); with shotnoise == 0 they stay
+ * mirrored.  theta0 offset per slice is 0.  Returns G4H_ERR_INVALID_ARGUMENT
+ * on bad sizes. */
+typedef struct g4h_beam_spec {
+  double gamma0, espread_rel, chirp_rel;
+  double emitn_x, emitn_y, betax, betay, alphax, alphay;
+  double xcen, ycen, pxcen, pycen;
+  double slicelength;
+  int32_t nslice, npart, nbins, shotnoise;
+  uint64_t seed;
+} g4h_beam_spec;
+g4h_status g4h_load_beam(const g4h_beam_spec* spec, const double* current, double* x,
+                         double* y, double* px, double* py, double* gamma, double* theta);
+
+/* Gaussian seed field at its waist: u = u0*exp(-r^2/w0^2) normalised to
+ * `power` [W] per slice (power = sum|u|^2 (dgrid eev/ks)^2/vacimp), written
+ * to storage planes re/im[nslice*ngrid^2]. */
+g4h_status g4h_seed_gaussian(int nslice, int ngrid, double gridmax, double xlambda,
+                             double power, double w0, double* re, double* im);
+
+/* A planar undulator beamline of nmod modules x nper periods (period
+ * lambdau, rms aw_mod[nmod]), one step per period, with FODO quadrupoles
+ * superimposed every quad_spacing_periods periods (length quad_len_periods,
+ * k1 alternating +k1/-k1, strength averaged over the step overlap), natural
+ * focusing ky = ku^2 (planar), slippage one reference length per step, an
+ * output every output_step steps.  Writes min(capacity, needed) steps and
+ * returns the count needed in *nsteps. */
+g4h_status g4h_lattice_fodo(double lambdau, const double* aw_mod, int nmod, int nper,
+                            int quad_spacing_periods, int quad_len_periods, double k1,
+                            int output_step, g4h_step* steps, int capacity, int* nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4HIP_H_ */

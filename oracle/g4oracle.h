@@ -1,0 +1,130 @@
+/*
+ * g4oracle.h - FP64 CPU restatement of the GENESIS 1.3 v4 time-dependent
+ * step as the reference CUDA overlay (mulingLHY/Genesis4_CUDA_port) computes
+ * it.  TEST INFRASTRUCTURE ONLY: this library is the checker for the HIP
+ * path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it; the product library never links or calls it.
+ *
+ * Every function names the reference file:line it follows.  All arrays are
+ * caller-owned host memory in FP64.
+ */
+#ifndef G4ORACLE_H_
+#define G4ORACLE_H_
+
+#include <stdint.h>
+#include "../include/g4hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One radiation field: storage-order planes (slice s of the ring lives in
+ * plane (s+first)%nslice), cell index ix + iy*ngrid (Field.cpp:187-201). */
+typedef struct orc_field {
+  int32_t nslice, ngrid, first, harm;
+  int32_t enabled, pad_;
+  double gridmax;      /* half width of the grid [m]            */
+  double dgrid;        /* 2*gridmax/(ngrid-1)                    */
+  double xks;          /* 2*pi/xlambda                           */
+  double xlambda;      /* wavelength of this harmonic [m]        */
+  double slicelength;  /* [m]                                    */
+  double accuslip;     /* accumulated slippage (Control state)   */
+  double* re;          /* nslice*ngrid*ngrid                     */
+  double* im;
+} orc_field;
+
+int orc_version(void);
+void orc_set_threads(int nthreads);
+
+/* Undulator coupling factor fc(h) (upstream Undulator::fc, called at
+ * BeamSolverCUDA.cpp:503 and FieldSolverADICUDA.cpp:707). */
+double orc_fc(double aw, int helical, int harm);
+
+/* TrackBeamCUDA::track (TrackBeamCUDA.cpp:156-204) including the chicane on
+ * the first half step and the corrector on the last (TrackBeamCUDA.cpp:181-185). */
+void orc_track_transverse(int64_t n, double* x, double* y, double* px, double* py,
+                          const double* gamma, const g4h_step* st, double gamma0,
+                          double delz, int last_step);
+
+/* TrackBeamCUDA::applyR56 (TrackBeamCUDA.cpp:257-278). */
+void orc_apply_r56(int64_t n, const double* gamma, double* theta, const g4h_step* st,
+                   double gamma0, double lambda0);
+
+/* BeamSolverCUDA::advance (BeamSolverCUDA.cpp:449-683): gather + RK4. */
+void orc_beam_advance(int nslice, const int32_t* off, const double* x, const double* y,
+                      const double* px, const double* py, double* gamma, double* theta,
+                      const orc_field* fields, int nfield, const g4h_step* st,
+                      double gamma0, double delz, const double* slice_eloss);
+
+/* Source construction (FieldSolverADICUDA.cpp:75-174, FFT twin
+ * FieldSolverFFTCUDA.cpp:77-155,383-395).  Writes full source planes in
+ * storage order (zeroed first).  Returns 1 if the source is on this step. */
+int orc_deposit(int nslice, const int32_t* off, const double* x, const double* y,
+                const double* gamma, const double* theta, const orc_field* f,
+                const g4h_step* st, const double* current, double delz,
+                double* src_re, double* src_im);
+
+/* Particle-to-cell indices (lower-left cell, or -1 when the particle is
+ * off the grid) exactly as the gather and the deposition compute them
+ * (BeamSolverCUDA.cpp:99-121, FieldSolverADICUDA.cpp:126-155). */
+void orc_cell_index(int64_t n, const double* x, const double* y, double gridmax,
+                    double dgrid, int ngrid, int32_t* cell);
+
+/* FieldSolverADICUDA::init Thomas factors (FieldSolverADICUDA.cpp:821-866). */
+void orc_adi_coefficients(int ngrid, double rtmp, double* c_re, double* c_im,
+                          double* cbet_re, double* cbet_im, double* cwet_re,
+                          double* cwet_im);
+
+/* FieldSolverADICUDA::advance field part (FieldSolverADICUDA.cpp:789-816)
+ * with Thomas sweeps (:222-277, :325-380).  src may be NULL. */
+void orc_adi_advance(orc_field* f, const double* src_re, const double* src_im,
+                     double delz);
+
+/* FieldSolverFFTCUDA::advance field part (FieldSolverFFTCUDA.cpp:413-481)
+ * with the k2/sigmoid tables of init (:273-323).  src may be NULL. */
+void orc_fft_advance(orc_field* f, const double* src_re, const double* src_im,
+                     double delz, int do_filter, double xc, double yc, double sig);
+
+/* Plain 1-D DFT used by the FFT solver restatement (forward = exp(-i...)). */
+void orc_dft(int n, const double* in_re, const double* in_im, double* out_re,
+             double* out_im, int inverse);
+
+/* ControlCUDA::applySlippage, single rank (ControlCUDA.cpp:148-252).
+ * Returns the number of ring shifts performed. */
+int orc_apply_slippage(orc_field* f, double slippage, double sample, int timerun,
+                       int periodic);
+
+/* Raw per-slice beam sums (DiagnosticCUDA.cpp:188-303); layout of
+ * g4hip_types.h, G4H_BEAM_RAW(nharm) doubles per slice.  gamma sums are taken
+ * relative to gamma_shift (see the .cpp). */
+void orc_diag_beam_raw(int nslice, const int32_t* off, const double* x, const double* y,
+                       const double* px, const double* py, const double* gamma,
+                       const double* theta, int nharm, double gamma_shift, double* raw);
+
+/* Raw per-slice field sums in logical slice order (DiagnosticCUDA.cpp:305-363). */
+void orc_diag_field_raw(const orc_field* f, double* raw);
+
+/* Host finalisation (DiagnosticCUDA.cpp:509-628 beam, :685-790 field). */
+void orc_finalize_beam(int nslice, int nharm, const double* raw, const double* current,
+                       double gamma_shift, double* q, double* global);
+void orc_finalize_field(int nslice, int ngrid, double dgrid, double xlambda,
+                        double slicelength, const double* raw, double* q,
+                        double* global);
+
+/* The Gencore::run step loop (Gencore.cpp:161-254, Beam::track Beam.cpp:
+ * 111-134, Field::track Field.cpp:204-212) for one field.  Records raw
+ * diagnostics before the loop and after every outstep.  beam_raw capacity:
+ * (1+#outsteps)*nslice*G4H_BEAM_RAW(nharm); field_raw: (1+#outsteps)*nslice*9.
+ * Returns the number of records written. */
+int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
+            double* py, double* gamma, double* theta, double gamma0, double lambda0,
+            const double* current, const double* slice_eloss, orc_field* field,
+            int solver, int do_filter, double xc, double yc, double sig,
+            const g4h_step* steps, int nsteps, double sample, int timerun,
+            int periodic, int nharm, double* beam_raw, double* field_raw);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4ORACLE_H_ */
