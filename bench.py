@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-slice-steps/s of the fused GENESIS 1.3 v4 time-dependent
+step on one MI355X.
+
+Workload (time-dependent SASE, FFT field solver): 4096 slices x 16384
+particles/slice, ngrid 256, gridmax 0.4 mm, Gaussian current (peak 4 kA,
+sigma 1000 slices), 1e-4 linear energy chirp, shot noise (seed 1234569),
+planar undulator lambdau 26 mm, aw 1.7 rms, gamma0 11350, FODO quadrupoles,
+diagnostics recorded every step.  Synthetic, seeded input; state resident in
+HBM before the timed region.
+
+value  = mixed-precision throughput (FP32 storage/push, FP64 phase, Thomas and
+         reductions); the FP64 path is timed on the same workload and reported
+         beside it with the mixed-vs-FP64 slice power / bunching error.
+A step = one lattice step of the whole beam: particle pass (transverse half
+steps, gather + RK4, deposition, beam diagnostics), FFT row and column
+passes (field diagnostics fused), slippage.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() \
+    else "particle-slice-steps/s"
+PEAK_HBM_GBS = 8000.0
+
+LAMBDAU, AW, GAMMA0 = 0.026, 1.7, 11350.0
+LAMBDA0 = LAMBDAU * (1 + AW * AW) / (2 * GAMMA0 * GAMMA0)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(nslice: int, npart: int):
+    from genesis4_cuda_port_amd import api
+    s = np.arange(nslice, dtype=np.float64)
+    current = 4000.0 * np.exp(-0.5 * ((s - 0.5 * (nslice - 1)) / 1000.0) ** 2)
+    beam = api.load_beam(nslice=nslice, npart=npart, nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+                         emitn=0.4e-6, beta=15.0, slicelength=LAMBDA0, current=current,
+                         shotnoise=True, seed=1234569, chirp_rel=1e-4)
+    steps = api.lattice_fodo(LAMBDAU, [AW] * 10, 100, 46, 3, 1.5, 1)
+    return beam, steps
+
+
+def algorithmic_bytes(cls: str, prec: int, np_: int, cells: int) -> float:
+    """Compulsory HBM bytes per launch of each kernel class (state read+write;
+    field gathers and the windowed source excluded)."""
+    sp = 8 if prec == 0 else 4     # particle scalar
+    sc = 16 if prec == 0 else 8    # complex cell
+    if cls == "particle":
+        return np_ * 2.0 * (5 * sp + 8)
+    if cls == "fft_rows":
+        return 2.0 * cells * sc
+    if cls in ("fft_cols", "adi_x", "adi_y"):
+        return 3.0 * cells * sc
+    return 0.0
+
+
+def run_gpu(prec: int, beam, steps, ngrid: int, gridmax: float, warmup: int, nsteps: int):
+    from genesis4_cuda_port_amd import api
+    from genesis4_cuda_port_amd._abi import KERNEL_CLASSES
+    nslice = beam.offsets.size - 1
+    with api.Context(0) as ctx:
+        b = api.Beam(ctx, prec, beam.offsets, beam.current, GAMMA0)
+        b.upload(beam.x, beam.y, beam.px, beam.py, beam.gamma, beam.theta)
+        f = api.Field(ctx, prec, nslice, ngrid, gridmax, LAMBDA0, LAMBDA0)
+        f.init_solver(api.FFT)
+        nrun = warmup + nsteps
+        run = api.Run(ctx, b, f, list(steps)[:nrun], LAMBDA0, 1.0, True, False, 1)
+        run.advance(warmup)
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        run.advance(nsteps)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        kern = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            n, ms = ctx.profile_read(k)
+            if n:
+                kern[name] = {"launches": n, "total_ms": ms, "avg_ms": ms / n}
+        ctx.profile(False)
+        braw, fraw = run.download_diag()
+        mem = ctx.memory()
+        run.close()
+        f.close()
+        b.close()
+    return (t1 - t0), kern, braw, fraw, mem
+
+
+def power_bunching(braw, fraw, current, ngrid, gridmax):
+    from genesis4_cuda_port_amd import api
+    dgrid = 2 * gridmax / (ngrid - 1)
+    fq, _ = api.finalize_field(fraw, dgrid, LAMBDA0, LAMBDA0)
+    bq, _ = api.finalize_beam(braw, 1, current, GAMMA0)
+    return fq[:, 0], bq[:, 25]
+
+
+def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int):
+    """The FP64 oracle on the host cores over the first nslice_cpu slices of the
+    same workload (one untimed step, then nsteps_cpu timed steps)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    _oracle.lib().orc_set_threads(threads)
+    sub = beam.copy()
+    npart = int(beam.offsets[1] - beam.offsets[0])
+    n = nslice_cpu * npart
+    for k in ("x", "y", "px", "py", "gamma", "theta"):
+        setattr(sub, k, getattr(beam, k)[:n].copy())
+    sub.offsets = beam.offsets[:nslice_cpu + 1].copy()
+    sub.current = beam.current[:nslice_cpu].copy()
+    of = _oracle.OField(nslice_cpu, ngrid, gridmax, LAMBDA0, LAMBDA0)
+    st = list(steps)
+    from genesis4_cuda_port_amd.api import steps_array
+    warm = steps_array(st[:1])
+    timed = steps_array(st[1:1 + nsteps_cpu])
+    sub, _, _ = _oracle.run(sub, of, warm, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
+    t0 = time.perf_counter()
+    _oracle.run(sub, of, timed, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
+    dt = time.perf_counter() - t0
+    return {"value": n * nsteps_cpu / dt, "unit": "particle-slice-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle, first {nslice_cpu} of 4096 slices x {npart} particles, "
+                      f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--slices", type=int, default=4096)
+    ap.add_argument("--particles", type=int, default=16384)
+    ap.add_argument("--ngrid", type=int, default=256)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=192)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 plain field copies of 4096 x 256^2 FP32 cells (2 GiB "
+                         "read + 2 GiB written each) to calibrate the HBM byte counters")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no "
+              "multi-GPU path.  The slice axis would shard across GPUs with one plane handed "
+              "over per slippage event; see the design notes.", file=sys.stderr)
+        return 2
+    from genesis4_cuda_port_amd import api
+    if api.device_count() < 1:
+        print("bench.py: no GPU visible to libg4hip", file=sys.stderr)
+        return 3
+    gridmax = 0.4e-3
+    if a.copy_calibration:
+        with api.Context(0) as ctx:
+            src = api.Field(ctx, api.MIXED, a.slices, a.ngrid, gridmax, LAMBDA0, LAMBDA0)
+            dst = api.Field(ctx, api.MIXED, a.slices, a.ngrid, gridmax, LAMBDA0, LAMBDA0)
+            for _ in range(3):
+                dst.copy_from(src)
+            ctx.synchronize()
+            nbytes = a.slices * a.ngrid * a.ngrid * 8
+            print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                              "copies": 3}), flush=True)
+        return 0
+    log(f"loading beam {a.slices} x {a.particles}")
+    t = time.perf_counter()
+    beam, steps = workload(a.slices, a.particles)
+    log(f"beam loaded in {time.perf_counter() - t:.1f} s")
+    np_total = a.slices * a.particles
+    cells = a.slices * a.ngrid * a.ngrid
+
+    log("mixed precision run")
+    tm, km, bm, fm, mem_m = run_gpu(api.MIXED, beam, steps, a.ngrid, gridmax, a.warmup, a.steps)
+    value = np_total * a.steps / tm
+    log(f"mixed: {tm / a.steps * 1e3:.3f} ms/step, {value:.3e} p-s-s/s")
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-slice-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": tm / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed: f32 state/field/push, f64 phase/Thomas/reductions)",
+        "data": "synthetic (seeded quiet-start beam with shot noise)",
+        "config": {"workload": "time-dependent SASE, FFT field solver, "
+                               f"{a.slices} slices x {a.particles} particles/slice, "
+                               f"ngrid {a.ngrid}, mixed precision",
+                   "nslice": a.slices, "npart_per_slice": a.particles, "ngrid": a.ngrid,
+                   "solver": "fft", "diagnostics": "every step", "parallelism": "single GPU"},
+    }
+    # roofline of the dominant kernel class of the mixed run
+    dom = max(km.items(), key=lambda kv: kv[1]["total_ms"])
+    name, st = dom
+    ab = algorithmic_bytes(name, api.MIXED, np_total, cells)
+    achieved = ab / (st["avg_ms"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": name, "achieved": achieved,
+                       "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                       "traffic": None, "algorithmic_bytes_per_launch": ab,
+                       "avg_launch_ms": st["avg_ms"]}
+    out["kernels_mixed"] = {k: dict(v, gbs=algorithmic_bytes(k, api.MIXED, np_total, cells) /
+                                    (v["avg_ms"] * 1e-3) / 1e9) for k, v in km.items()}
+    out["hbm_bytes_resident_mixed"] = mem_m
+
+    if not a.no_fp64:
+        log("FP64 run")
+        t64, k64, b64, f64, mem_d = run_gpu(api.FP64, beam, steps, a.ngrid, gridmax, a.warmup,
+                                            a.steps)
+        v64 = np_total * a.steps / t64
+        log(f"fp64: {t64 / a.steps * 1e3:.3f} ms/step, {v64:.3e} p-s-s/s")
+        pm, bmq = power_bunching(bm[-1].ravel(), fm[-1].ravel(), beam.current, a.ngrid, gridmax)
+        pd, bdq = power_bunching(b64[-1].ravel(), f64[-1].ravel(), beam.current, a.ngrid, gridmax)
+        out["fp64"] = {"value": v64, "ms_per_step": t64 / a.steps * 1e3,
+                       "kernels": {k: dict(v, gbs=algorithmic_bytes(k, api.FP64, np_total, cells) /
+                                           (v["avg_ms"] * 1e-3) / 1e9) for k, v in k64.items()}}
+        out["mixed_over_fp64"] = value / v64
+        out["mixed_vs_fp64_error"] = {
+            "power_rel": float(np.linalg.norm(pm - pd) / max(np.linalg.norm(pd), 1e-300)),
+            "bunching_rel": float(np.linalg.norm(bmq - bdq) / max(np.linalg.norm(bdq), 1e-300)),
+            "at_step": a.warmup + a.steps}
+    if not a.no_cpu:
+        log("cpu baseline (FP64 oracle)")
+        out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
+                                           a.cpu_steps)
+    out["latency_p50_p99"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -104,6 +104,7 @@ PROTOTYPES = {
     "g4h_field_destroy": (None, [_vp]),
     "g4h_field_upload": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "g4h_field_download": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_int)]),
+    "g4h_field_copy": (C.c_int, [_vp, _vp, _vp]),
     "g4h_field_get_ring": (C.c_int, [_vp, C.POINTER(C.c_int), _dp]),
     "g4h_field_set_ring": (C.c_int, [_vp, C.c_int, C.c_double]),
     "g4h_field_set_enabled": (C.c_int, [_vp, C.c_int]),

@@ -196,6 +196,9 @@ class Field:
         check(lib().g4h_field_download(self.h, _dp(re), _dp(im), C.byref(first)))
         return re, im, int(first.value)
 
+    def copy_from(self, src: "Field"):
+        check(lib().g4h_field_copy(self.ctx.h, self.h, src.h))
+
     def ring(self):
         first = C.c_int()
         acc = C.c_double()

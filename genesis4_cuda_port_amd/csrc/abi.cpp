@@ -554,6 +554,16 @@ g4h_status g4h_field_download(g4h_field* f, double* re, double* im, int* first) 
   G4_API_END
 }
 
+g4h_status g4h_field_copy(g4h_context* ctx, g4h_field* dst, const g4h_field* src) {
+  G4_API_BEGIN
+  require(ctx && dst && src, "null argument");
+  require(dst->prec == src->prec && dst->nslice == src->nslice && dst->ngrid == src->ngrid,
+          "fields differ in precision or shape");
+  ctx->activate();
+  if (dst != src) field_copy(dst, src);
+  G4_API_END
+}
+
 g4h_status g4h_field_get_ring(const g4h_field* f, int* first, double* accuslip) {
   G4_API_BEGIN
   require(f, "field is null");

@@ -87,6 +87,13 @@ __global__ __launch_bounds__(256) void plane_stats_kernel(const T2* u, int G, do
   }
 }
 
+// Streaming copy, 16 bytes per lane per iteration.
+__global__ void copy16_kernel(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void reset_rect_kernel(int4* rect, int4* bbox, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -195,6 +202,20 @@ void field_zero_plane(Field* f, int slot) {
       hipLaunchKernelGGL(zero_plane_kernel<float2>, dim3(blocks), dim3(256), 0, f->ctx->stream,
                          f->uf.get(), plane, slot, f->zeroed.get());
   });
+}
+
+void field_copy(Field* dst, const Field* src) {
+  const size_t bytes = static_cast<size_t>(src->cells()) * (src->prec == G4H_FP64 ? 16 : 8);
+  const int64_t n16 = static_cast<int64_t>(bytes / 16);
+  const int blocks = static_cast<int>(std::min<int64_t>(8192, (n16 + 255) / 256));
+  dst->ctx->launch(G4H_K_OTHER, [&] {
+    hipLaunchKernelGGL(copy16_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, dst->ctx->stream,
+                       static_cast<const float4*>(src->data()), static_cast<float4*>(dst->data()),
+                       n16);
+  });
+  dst->first = src->first;
+  dst->accuslip = src->accuslip;
+  dst->uploaded = true;
 }
 
 void field_map_stats(Field* f, double* d_raw) {

@@ -3,11 +3,14 @@
 // these from upstream GENESIS 1.3 (Undulator, QuietLoading, ShotNoise,
 // LoadField, Lattice), which is not part of it; each piece below is restated
 // from the published method named in its comment.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/g4hip.h"
@@ -114,13 +117,16 @@ g4h_status g4h_sobol(int dims, int64_t index0, int64_t n, double* out) {
 
 // Inverse standard normal CDF: Halley iteration on Phi(x) - p with
 // Phi(x) = erfc(-x/sqrt2)/2, started from the Tukey-lambda approximation
-// x0 = 4.91 (p^0.14 - (1-p)^0.14).
+// x0 = 4.91 (p^0.14 - (1-p)^0.14).  The upper half uses Phi^-1(p) =
+// -Phi^-1(1-p) (1-p is exact there) so both tails are solved where erfc is
+// accurate.
 double g4h_inverse_normal(double p) {
   if (!(p > 0.0 && p < 1.0)) {
     if (p == 0.0) return -HUGE_VAL;
     if (p == 1.0) return HUGE_VAL;
     return NAN;
   }
+  if (p > 0.5) return -g4h_inverse_normal(1.0 - p);
   double x = 4.91 * (std::pow(p, 0.14) - std::pow(1.0 - p, 0.14));
   for (int it = 0; it < 60; ++it) {
     const double phi = 0.5 * std::erfc(-x / std::sqrt(2.0));
@@ -154,8 +160,12 @@ g4h_status g4h_load_beam(const g4h_beam_spec* sp, const double* current, double*
   const double sx = std::sqrt(ex * sp->betax), sxp = std::sqrt(ex / sp->betax);
   const double sy = std::sqrt(ey * sp->betay), syp = std::sqrt(ey / sp->betay);
   const double twopi = 2.0 * kPi;
+  // Slices are independent (Sobol index and RNG stream are per slice), so
+  // they are loaded by a pool of host threads; the result does not depend on
+  // the thread count.
+  auto load_slices = [&](int s0, int s1) {
   std::vector<double> th0(sp->nbins);
-  for (int is = 0; is < sp->nslice; ++is) {
+  for (int is = s0; is < s1; ++is) {
     const double chirp =
         sp->nslice > 1 ? sp->chirp_rel * sp->gamma0 * (static_cast<double>(is) / (sp->nslice - 1) - 0.5) : 0.0;
     std::mt19937_64 rn(splitmix64(sp->seed ^ (0x5851F42D4C957F2Dull * static_cast<uint64_t>(is + 1))));
@@ -202,6 +212,18 @@ g4h_status g4h_load_beam(const g4h_beam_spec* sp, const double* current, double*
       }
     }
   }
+  };
+  int nthr = static_cast<int>(std::thread::hardware_concurrency());
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) nthr = std::atoi(e);
+  nthr = std::max(1, std::min({nthr, 64, sp->nslice}));
+  if (static_cast<int64_t>(sp->nslice) * sp->npart < (1 << 16)) nthr = 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthr; ++t) {
+    const int s0 = static_cast<int>(static_cast<int64_t>(sp->nslice) * t / nthr);
+    const int s1 = static_cast<int>(static_cast<int64_t>(sp->nslice) * (t + 1) / nthr);
+    pool.emplace_back(load_slices, s0, s1);
+  }
+  for (auto& th : pool) th.join();
   return G4H_OK;
 }
 

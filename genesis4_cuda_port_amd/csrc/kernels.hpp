@@ -94,6 +94,7 @@ void field_zero_plane(Field* f, int slot);             // slippage zero fill
 void field_map_stats(Field* f, double* d_raw);
 // source bookkeeping reset (new beam or new source allocation)
 void field_reset_source(Field* f);
+void field_copy(Field* dst, const Field* src);
 void ensure_source(Field* f);
 
 }  // namespace g4

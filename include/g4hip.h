@@ -94,6 +94,11 @@ void g4h_field_destroy(g4h_field* field);
 g4h_status g4h_field_upload(g4h_field* field, const double* re, const double* im,
                             int first);
 g4h_status g4h_field_download(g4h_field* field, double* re, double* im, int* first);
+/* Device-to-device copy of the planes and ring state of `src` into `dst`
+ * (same precision, nslice and ngrid): a state snapshot / clone with one
+ * streaming copy kernel.  No counterpart in the reference, where the only
+ * copy of the field is the host-side pack/unpack (Field.cpp:598-674). */
+g4h_status g4h_field_copy(g4h_context* ctx, g4h_field* dst, const g4h_field* src);
 /* Ring state used by the slippage (Field::first, Field::accuslip). */
 g4h_status g4h_field_get_ring(const g4h_field* field, int* first, double* accuslip);
 g4h_status g4h_field_set_ring(g4h_field* field, int first, double accuslip);

@@ -1,0 +1,567 @@
+"""Pin the FP64 oracle against things other than itself (CPU only).
+
+Each test checks one oracle function against an independent source: a
+textbook closed form, an independent numerical method (scipy ODE solver,
+numpy dense linear algebra, numpy FFT) or a physical invariant, chosen so that
+a plausible mistake (dropped term, wrong sign, transposed operand, wrong
+index) fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy import special
+from scipy.integrate import solve_ivp
+
+import _oracle
+from _oracle import OField, dp, ip
+import cases
+from cases import AW, GAMMA0, LAMBDA0, LAMBDAU, quad_step
+from genesis4_cuda_port_amd import api
+
+L = None
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _lib():
+    global L
+    L = _oracle.lib()
+    yield
+
+
+# ---------------------------------------------------------------------------
+# undulator coupling
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("aw", [0.5, 1.0, 1.7, 2.613])
+@pytest.mark.parametrize("h", [1, 3, 5])
+def test_fc_matches_bessel_closed_form(aw, h):
+    xi = h * aw * aw / (2 * (1 + aw * aw))
+    n = (h - 1) // 2
+    want = aw * (-1) ** n * (special.jv(n, xi) - special.jv(n + 1, xi))
+    assert L.orc_fc(aw, 0, h) == pytest.approx(want, rel=1e-13)
+    assert api.undulator_fc(aw, False, h) == pytest.approx(want, rel=1e-13)
+
+
+def test_fc_helical_and_even():
+    assert L.orc_fc(1.3, 1, 1) == 1.3 and L.orc_fc(1.3, 1, 3) == 0.0
+    assert L.orc_fc(1.3, 0, 2) == 0.0 and L.orc_fc(0.0, 0, 1) == 0.0
+    # textbook planar value K = sqrt(2)*1.7: JJ = J0(xi) - J1(xi) = 0.7833 (xi = 0.3715)
+    assert L.orc_fc(1.7, 0, 1) / 1.7 == pytest.approx(0.78325, abs=2e-5)
+
+
+# ---------------------------------------------------------------------------
+# transverse maps
+# ---------------------------------------------------------------------------
+def _track(x, px, gamma, st, dz, last=False):
+    n = x.size
+    x = x.copy()
+    px = px.copy()
+    y = np.zeros(n)
+    py = np.zeros(n)
+    L.orc_track_transverse(n, dp(x), dp(y), dp(px), dp(py), dp(np.ascontiguousarray(gamma)), st,
+                           GAMMA0, dz, int(last))
+    return x, px
+
+
+@pytest.mark.parametrize("qf", [1.5, -1.5])
+def test_quad_map_matches_ode(qf):
+    """A particle at gamma0 in a quad of k1 = qf: x'' = -k1 (gamma0/gammaz) x,
+    integrated with an independent solver."""
+    st = quad_step(qf=qf, aw=0.0, in_und=False)
+    dz = 0.05
+    x0, p0 = np.array([1e-5]), np.array([2e-2])
+    g = np.array([GAMMA0])
+    x1, p1 = _track(x0, p0, g, st, dz)
+    gz = np.sqrt(GAMMA0 ** 2 - 1 - p0[0] ** 2)
+    k = qf * GAMMA0 / gz
+
+    def rhs(z, s):
+        return [s[1] / gz, -k * gz * s[0]]
+
+    sol = solve_ivp(rhs, (0, dz), [x0[0], p0[0]], method="DOP853", rtol=1e-13, atol=1e-20)
+    assert x1[0] == pytest.approx(sol.y[0, -1], rel=1e-9)
+    assert p1[0] == pytest.approx(sol.y[1, -1], rel=1e-9)
+
+
+def test_natural_focusing_wavenumber():
+    """Planar undulator: vertical betatron wavenumber aw*ku/gamma (K = sqrt2 aw)."""
+    st = quad_step()  # ky = ku^2, aw = 1.7
+    n = 1
+    y0 = np.array([1e-5])
+    dz = 2.0
+    x = np.zeros(n)
+    px = np.zeros(n)
+    y = y0.copy()
+    py = np.zeros(n)
+    L.orc_track_transverse(n, dp(x), dp(y), dp(px), dp(py), dp(np.array([GAMMA0])), st, GAMMA0,
+                           dz, 0)
+    ku = 2 * np.pi / LAMBDAU
+    kb = AW * ku / GAMMA0
+    assert y[0] == pytest.approx(y0[0] * np.cos(kb * dz), rel=1e-6)
+    assert x[0] == 0.0
+
+
+def test_transverse_maps_symplectic():
+    """Finite-difference Jacobian of (x, px) over focusing, defocusing and
+    drift half steps has unit determinant (area preserving)."""
+    for qf, aw in ((1.5, AW), (-1.5, AW), (0.0, 0.0)):
+        st = quad_step(qf=qf, aw=aw, in_und=aw > 0)
+        g = np.full(3, GAMMA0)
+        h = 1e-9
+        x = np.array([1e-5, 1e-5 + h, 1e-5])
+        p = np.array([1e-2, 1e-2, 1e-2 + h * 1e3])
+        x1, p1 = _track(x, p, g, st, 0.013)
+        J = np.array([[(x1[1] - x1[0]) / h, (x1[2] - x1[0]) / (h * 1e3)],
+                      [(p1[1] - p1[0]) / h, (p1[2] - p1[0]) / (h * 1e3)]])
+        assert np.linalg.det(J) == pytest.approx(1.0, abs=1e-5)
+
+
+def test_chicane_near_identity_and_r56_small_angle():
+    """With its backward drift the chicane map is identity + O(angle^2), and
+    R56 -> 2 angle^2 (ld + 2 lb/3) for small angles."""
+    ang, lb, ld, lt = 1e-3, 0.1, 0.5, 2.0
+    st = quad_step(aw=0.0, in_und=False, chicane=(ang, lb, ld, lt))
+    # map via zero-length tracking: the drift part is done with delz = 0
+    x, px = _track(np.array([1e-4, 0.0]), np.array([0.0, 1e-2]), np.full(2, GAMMA0), st, 0.0)
+    assert abs(x[0] - 1e-4) < 10 * ang ** 2 * 1e-4 / lb
+    gz = np.sqrt(GAMMA0 ** 2 - 1 - 1e-4)
+    assert abs(x[1]) < 10 * ang ** 2 * 1e-2 / gz * lt
+    n = 1
+    th = np.zeros(n)
+    g = np.array([GAMMA0 + 1.0])
+    L.orc_apply_r56(n, dp(g), dp(th), st, GAMMA0, LAMBDA0)
+    r56 = th[0] / (2 * np.pi / LAMBDA0 / GAMMA0 * 1.0)
+    assert r56 == pytest.approx(2 * ang ** 2 * (ld + 2 * lb / 3), rel=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# RK4 push
+# ---------------------------------------------------------------------------
+def _push(c, field_re, field_im, st, dz, eloss=None):
+    o = c.beam.copy()
+    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, field_re, field_im)
+    arr = (_oracle.OrcField * 1)(of.s)
+    arr[0].re = dp(of.re)
+    arr[0].im = dp(of.im)
+    L.orc_beam_advance(c.nslice, ip(o.offsets), dp(o.x), dp(o.y), dp(o.px), dp(o.py),
+                       dp(o.gamma), dp(o.theta), arr, 1, st, GAMMA0, dz,
+                       None if eloss is None else dp(np.ascontiguousarray(eloss)))
+    return o
+
+
+def test_push_zero_field_closed_form():
+    c = cases.make_case(nslice=2, npart=64, ngrid=9, shotnoise=False)
+    st = quad_step(autophase=0.25)
+    z = np.zeros(c.nslice * c.ngrid ** 2)
+    el = np.array([1e-3, -2e-3])
+    o = _push(c, z, z, st, st.delz, el)
+    b = c.beam
+    xks = 2 * np.pi / LAMBDA0
+    ku = 2 * np.pi / LAMBDAU
+    awloc = 1 + 0.5 * (ku ** 2) * b.y ** 2
+    btpar = 1 + b.px ** 2 + b.py ** 2 + AW ** 2 * awloc ** 2
+    ez = np.repeat(el, 64)
+    # gamma is linear in z; theta' depends on gamma -> integrate exactly in z
+    g0 = b.gamma
+    zs = np.linspace(0, st.delz, 2001)
+    th_rate = [xks * (1 - 1 / np.sqrt(1 - btpar / (g0 - ez * z) ** 2)) + ku for z in zs]
+    dth = np.trapezoid(np.array(th_rate), zs, axis=0)
+    assert np.allclose(o.gamma, g0 - ez * st.delz, rtol=0, atol=1e-10)
+    assert np.allclose(o.theta, b.theta + 0.25 + dth, rtol=0, atol=1e-7)
+
+
+def test_push_resonant_particle_is_stationary():
+    """An on-axis particle at the resonant energy keeps its phase: theta' ~ 0."""
+    c = cases.make_case(nslice=1, npart=4, ngrid=9, shotnoise=False)
+    o0 = c.beam
+    o0.x[:] = 0
+    o0.y[:] = 0
+    o0.px[:] = 0
+    o0.py[:] = 0
+    o0.gamma[:] = GAMMA0
+    st = quad_step()
+    z = np.zeros(c.nslice * c.ngrid ** 2)
+    o = _push(c, z, z, st, 1.0)
+    assert np.all(np.abs(o.theta - o0.theta) < 1e-5)
+
+
+def test_rk4_fourth_order_in_a_constant_field():
+    """Pendulum motion in a uniform field: the oracle's RK4 converges at 4th
+    order to an independent high-accuracy solution."""
+    c = cases.make_case(nslice=1, npart=4, ngrid=9, shotnoise=False)
+    b = c.beam
+    b.x[:] = 0
+    b.y[:] = 0
+    b.px[:] = 0
+    b.py[:] = 0
+    # strong field and 2% detuning: truncation error well above the FP64
+    # floor of theta' (xks * (1 - 1/beta_par) cancels ~8 digits)
+    b.gamma[:] = GAMMA0 * (1 + 0.02 * np.array([-1.0, -0.5, 0.5, 1.0]))
+    b.theta[:] = np.array([0.3, 1.1, 2.5, 4.0])
+    u0 = 3e13
+    re = np.full(c.ngrid ** 2, u0)
+    im = np.full(c.ngrid ** 2, 0.5 * u0)
+    Z = 1.0
+    xks = 2 * np.pi / LAMBDA0
+    ku = 2 * np.pi / LAMBDAU
+    fc = L.orc_fc(AW, 0, 1)
+    rr, ri = fc / xks * re[0], -fc / xks * im[0]
+    btpar = 1 + AW ** 2
+
+    def rhs(z, s):
+        g, t = s[:4], s[4:]
+        c_, s_ = np.cos(t), np.sin(t)
+        cre = rr * c_ + ri * s_
+        cim = ri * c_ - rr * s_
+        bp = np.sqrt(1 - (btpar - 2 / xks * cre) / g ** 2)
+        return np.concatenate([cim / bp / g, xks * (1 - 1 / bp) + ku])
+
+    ref = solve_ivp(rhs, (0, Z), np.concatenate([b.gamma, b.theta]), method="DOP853",
+                    rtol=1e-13, atol=1e-12)
+    yref = ref.y[:, -1]
+    errs = []
+    for n in (4, 8, 16):
+        h = Z / n
+        st = quad_step(delz=h, ky=0.0)
+        cc = cases.make_case(nslice=1, npart=4, ngrid=9, shotnoise=False)
+        cc.beam = b.copy()
+        for _ in range(n):
+            o = _push(cc, re, im, st, h)
+            cc.beam = o
+        errs.append(np.max(np.abs(np.concatenate([o.gamma - yref[:4], o.theta - yref[4:]]))))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 13 < r1 < 22 and 13 < r2 < 22, errs
+
+
+# ---------------------------------------------------------------------------
+# deposition
+# ---------------------------------------------------------------------------
+def test_deposit_known_answers_and_partition_of_unity():
+    c = cases.make_case(nslice=3, npart=400, ngrid=17, shotnoise=True)
+    st = quad_step()
+    b = c.beam
+    gm = c.gridmax
+    dg = 2 * gm / (c.ngrid - 1)
+    # particle 0 at the centre of cell (5, 7): four equal weights
+    b.x[0] = -gm + 5.5 * dg
+    b.y[0] = -gm + 7.5 * dg
+    of = OField(c.nslice, c.ngrid, gm, LAMBDA0, c.slicelength, 1, first=2)
+    plane = c.ngrid ** 2
+    sre = np.zeros(c.nslice * plane)
+    sim = np.zeros_like(sre)
+    on = L.orc_deposit(c.nslice, ip(b.offsets), dp(b.x), dp(b.y), dp(b.gamma), dp(b.theta),
+                       of.ptr, st, dp(b.current), st.delz, dp(sre), dp(sim))
+    assert on == 1
+    xks = 2 * np.pi / LAMBDA0
+    fc = L.orc_fc(AW, 0, 1)
+    ku = 2 * np.pi / LAMBDAU
+    scl = fc * 376.73 * b.current * xks * st.delz / (4 * 510999.06 * 400 * dg * dg)
+    for s in range(c.nslice):
+        sl = slice(b.offsets[s], b.offsets[s + 1])
+        faw = np.sqrt(1 + ku ** 2 * b.y[sl] ** 2)
+        part = faw * scl[s] / b.gamma[sl]
+        tot = np.sum(part * np.sin(b.theta[sl])) + 1j * np.sum(part * np.cos(b.theta[sl]))
+        t = (s + 2) % c.nslice  # storage plane
+        got = sre[t * plane:(t + 1) * plane].sum() + 1j * sim[t * plane:(t + 1) * plane].sum()
+        assert got == pytest.approx(tot, rel=1e-12)
+    # single-particle check: remove particle 0 and compare the difference
+    b2 = b.copy()
+    b2.x[0] = 10.0  # off grid
+    sre2 = np.zeros_like(sre)
+    sim2 = np.zeros_like(sre)
+    L.orc_deposit(c.nslice, ip(b2.offsets), dp(b2.x), dp(b2.y), dp(b2.gamma), dp(b2.theta),
+                  of.ptr, st, dp(b2.current), st.delz, dp(sre2), dp(sim2))
+    d = (sre - sre2)[2 * plane:3 * plane].reshape(c.ngrid, c.ngrid)
+    nz = np.argwhere(np.abs(d) > 0)
+    assert sorted(map(tuple, nz)) == [(7, 5), (7, 6), (8, 5), (8, 6)]
+    vals = d[d != 0]
+    assert np.allclose(vals, vals[0], rtol=1e-12)
+
+
+def test_cell_index_rule():
+    """Off-grid and boundary rule of the gather/deposition index."""
+    gm, ng = 1.0, 11
+    dg = 2 * gm / (ng - 1)
+    x = np.array([-gm, gm, -gm + 1e-12, gm - 1e-12, 0.0, -gm + 9.5 * dg, 2.0,
+                  np.nextafter(gm, 0.0)])
+    y = np.zeros_like(x)
+    out = np.empty(x.size, dtype=np.int32)
+    L.orc_cell_index(x.size, dp(x), dp(y), gm, dg, ng, ip(out))
+    assert out[0] == -1 and out[1] == -1 and out[6] == -1  # |x| >= gridmax is off grid
+    assert out[2] == 0 + 5 * ng
+    assert out[3] == 9 + 5 * ng   # last lower-left cell is ngrid-2
+    assert out[4] == 5 + 5 * ng
+    assert out[5] == 9 + 5 * ng
+    # one ulp inside +gridmax the quotient rounds to ngrid-1: not a lower-left cell
+    assert (x[7] + gm) / dg == ng - 1 and out[7] == -1
+
+
+# ---------------------------------------------------------------------------
+# field solvers
+# ---------------------------------------------------------------------------
+def _adi_dense(u, s, rtmp):
+    G = u.shape[0]
+    D = np.diag(-2 * np.ones(G)) + np.diag(np.ones(G - 1), 1) + np.diag(np.ones(G - 1), -1)
+    I = np.eye(G)
+    A = I - 1j * rtmp * D
+    B = I + 1j * rtmp * D
+    # u[iy, ix]; step A: x implicit, y explicit
+    r = B @ u + s          # (1 + i r d2y) u acts on the row index
+    ustar = np.linalg.solve(A, r.T).T  # solve along x for every row
+    r2 = (B @ ustar.T).T + s
+    return np.linalg.solve(A, r2)
+
+
+@pytest.mark.parametrize("G", [6, 9])
+def test_adi_matches_dense_solve(G):
+    rng = np.random.default_rng(G)
+    u = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+    s = 0.1 * (rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G)))
+    gm = 1e-4
+    of = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, u.real.ravel(), u.imag.ravel())
+    delz = 0.5
+    L.orc_adi_advance(of.ptr, dp(np.ascontiguousarray(s.real.ravel())),
+                      dp(np.ascontiguousarray(s.imag.ravel())), delz)
+    dg = 2 * gm / (G - 1)
+    rtmp = 0.25 * delz / (2 * np.pi / LAMBDA0 * dg * dg)
+    want = _adi_dense(u, s, rtmp)
+    got = (of.re + 1j * of.im).reshape(G, G)
+    assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_adi_coefficients_factor_the_matrix():
+    """Thomas factors: L U of the tridiagonal matrix reproduces it."""
+    G, r = 7, 0.3
+    c = [np.empty(G) for _ in range(6)]
+    L.orc_adi_coefficients(G, r, *[dp(a) for a in c])
+    cl = c[0] + 1j * c[1]
+    cbet = c[2] + 1j * c[3]
+    cwet = c[4] + 1j * c[5]
+    D = np.diag(-2 * np.ones(G)) + np.diag(np.ones(G - 1), 1) + np.diag(np.ones(G - 1), -1)
+    A = np.eye(G) - 1j * r * D
+    # forward elimination of A x = e_k must equal the dense inverse
+    for k in range(G):
+        rhs = np.zeros(G, complex)
+        rhs[k] = 1
+        u = np.zeros(G, complex)
+        u[0] = rhs[0] * cbet[0]
+        for i in range(1, G):
+            u[i] = (rhs[i] - cl[i] * u[i - 1]) * cbet[i]
+        for i in range(G - 2, -1, -1):
+            u[i] -= cwet[i + 1] * u[i + 1]
+        assert np.allclose(A @ u, rhs, atol=1e-14)
+
+
+def test_adi_and_fft_conserve_power_without_source():
+    c = cases.make_case(nslice=2, npart=4, ngrid=31, seed_power=1e6, w0=30e-6)
+    for solver in ("adi", "fft"):
+        rng = np.random.default_rng(1)
+        re = c.seed_re * (1 + 0.1 * rng.standard_normal(c.seed_re.size))
+        im = 0.05 * c.seed_re * rng.standard_normal(c.seed_re.size)
+        of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, re, im)
+        p0 = np.sum(re ** 2 + im ** 2)
+        for _ in range(5):
+            if solver == "adi":
+                L.orc_adi_advance(of.ptr, None, None, 0.5)
+            else:
+                L.orc_fft_advance(of.ptr, None, None, 0.5, 0, 1.0, 1.0, 1.0)
+        assert np.sum(of.re ** 2 + of.im ** 2) == pytest.approx(p0, rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 31, 64])
+def test_dft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ore, oim = np.empty(n), np.empty(n)
+    L.orc_dft(n, dp(np.ascontiguousarray(a.real)), dp(np.ascontiguousarray(a.imag)), dp(ore),
+              dp(oim), 0)
+    assert np.allclose(ore + 1j * oim, np.fft.fft(a), rtol=0, atol=1e-12 * n)
+    L.orc_dft(n, dp(np.ascontiguousarray(a.real)), dp(np.ascontiguousarray(a.imag)), dp(ore),
+              dp(oim), 1)
+    assert np.allclose(ore + 1j * oim, np.fft.ifft(a) * n, rtol=0, atol=1e-12 * n)
+
+
+def test_fft_step_source_is_added_twice_and_identity_at_zero_length():
+    G = 16
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(G * G) + 1j * rng.standard_normal(G * G)
+    s = rng.standard_normal(G * G) + 1j * rng.standard_normal(G * G)
+    a = OField(1, G, 1e-4, LAMBDA0, LAMBDA0, 1, u.real, u.imag)
+    b = OField(1, G, 1e-4, LAMBDA0, LAMBDA0, 1, u.real, u.imag)
+    L.orc_fft_advance(a.ptr, dp(np.ascontiguousarray(s.real)), dp(np.ascontiguousarray(s.imag)),
+                      0.3, 0, 1.0, 1.0, 1.0)
+    L.orc_fft_advance(b.ptr, None, None, 0.3, 0, 1.0, 1.0, 1.0)
+    assert np.allclose((a.re - b.re) + 1j * (a.im - b.im), 2 * s, atol=1e-12)
+    z = OField(1, G, 1e-4, LAMBDA0, LAMBDA0, 1, u.real, u.imag)
+    L.orc_fft_advance(z.ptr, None, None, 0.0, 0, 1.0, 1.0, 1.0)
+    assert np.allclose(z.re + 1j * z.im, u, atol=1e-13)
+
+
+def _gaussian_rms(of, G, gm):
+    p = (of.re ** 2 + of.im ** 2).reshape(G, G)
+    xs = np.linspace(-gm, gm, G)
+    return np.sqrt(np.sum(p.sum(0) * xs ** 2) / p.sum())
+
+
+@pytest.mark.parametrize("solver", ["adi", "fft"])
+def test_gaussian_beam_diffraction(solver):
+    """Free-space Gaussian mode: rms size grows as w0/2 sqrt(1 + (z/zR)^2)."""
+    G, gm, w0 = 81, 0.4e-3, 40e-6
+    re, im = api.seed_gaussian(1, G, gm, LAMBDA0, 1.0, w0)
+    of = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, re, im)
+    zr = np.pi * w0 ** 2 / LAMBDA0
+    z, nstep = zr, 40
+    for _ in range(nstep):
+        if solver == "adi":
+            L.orc_adi_advance(of.ptr, None, None, z / nstep)
+        else:
+            L.orc_fft_advance(of.ptr, None, None, z / nstep, 0, 1.0, 1.0, 1.0)
+    want = w0 / 2 * np.sqrt(1 + (z / zr) ** 2)
+    assert _gaussian_rms(of, G, gm) == pytest.approx(want, rel=0.02)
+
+
+def test_adi_and_fft_agree_on_a_smooth_beam():
+    """Both discretise du/dz = (i/2k) lap u: on a well-resolved Gaussian they
+    agree to the ADI's O(dgrid^2) Laplacian error, not to a sign."""
+    G, gm, w0 = 63, 0.3e-3, 50e-6
+    re, im = api.seed_gaussian(1, G, gm, LAMBDA0, 1.0, w0)
+    a = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, re, im)
+    f = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, re, im)
+    for _ in range(20):
+        L.orc_adi_advance(a.ptr, None, None, 0.5)
+        L.orc_fft_advance(f.ptr, None, None, 0.5, 0, 1.0, 1.0, 1.0)
+    ua = a.re + 1j * a.im
+    uf = f.re + 1j * f.im
+    assert np.linalg.norm(ua - uf) / np.linalg.norm(uf) < 2e-2
+    u0 = re + 1j * im
+    assert np.linalg.norm(uf - u0) / np.linalg.norm(u0) > 0.1  # it did propagate
+
+
+# ---------------------------------------------------------------------------
+# slippage
+# ---------------------------------------------------------------------------
+def test_slippage_integer_model():
+    S, G = 7, 3
+    re = np.arange(S * G * G, dtype=float) + 1
+    of = OField(S, G, 1e-4, LAMBDA0, LAMBDA0, 1, re, np.zeros_like(re), first=3)
+    zeroed = []
+    for k in range(1, 10):
+        n = L.orc_apply_slippage(of.ptr, 1.0, 1.0, 1, 0)
+        assert n == 1
+        assert of.first == (3 - k) % S
+        zeroed.append(of.first)
+        for z in zeroed[-S:]:
+            assert np.all(of.re.reshape(S, G * G)[z] == 0)
+    # periodic: nothing is zeroed, the ring only rotates
+    of2 = OField(S, G, 1e-4, LAMBDA0, LAMBDA0, 1, re, np.zeros_like(re), first=0)
+    assert L.orc_apply_slippage(of2.ptr, 2.5, 1.0, 1, 1) == 2
+    assert of2.first == S - 2 and np.array_equal(of2.re, re)
+    assert of2.s.accuslip == pytest.approx(0.5)
+    # steady state: untouched
+    assert L.orc_apply_slippage(of2.ptr, 5.0, 1.0, 0, 0) == 0
+
+
+# ---------------------------------------------------------------------------
+# diagnostics
+# ---------------------------------------------------------------------------
+def test_quiet_start_has_no_bunching_and_matched_moments():
+    c = cases.make_case(nslice=4, npart=4096, shotnoise=False)
+    b = c.beam
+    raw = np.zeros(4 * (23 + 2 * 3))
+    L.orc_diag_beam_raw(4, ip(b.offsets), dp(b.x), dp(b.y), dp(b.px), dp(b.py), dp(b.gamma),
+                        dp(b.theta), 3, GAMMA0, dp(raw))
+    q, g = api.finalize_beam(raw, 3, b.current, GAMMA0)
+    assert np.all(q[:, 25] < 1e-12) and np.all(q[:, 27] < 1e-12) and np.all(q[:, 29] < 1e-12)
+    eps = cases.EMITN / GAMMA0
+    assert np.allclose(q[:, 3], np.sqrt(eps * cases.BETA), rtol=0.02)        # xsize
+    assert np.allclose(q[:, 9], cases.EMITN, rtol=0.03)                     # emitx (normalised)
+    assert np.allclose(q[:, 11], cases.BETA, rtol=0.05)                     # betax
+    assert np.allclose(q[:, 1], cases.ESPREAD * GAMMA0, rtol=0.02)          # energy spread
+    assert np.allclose(q[:, 0], GAMMA0, rtol=1e-6)
+
+
+def test_shot_noise_bunching_statistics():
+    """Each slice's |b|^2 averages to 1/Ne (Ne electrons in the slice)."""
+    S, npart = 400, 256
+    sample = 2000.0  # long slices: many electrons per beamlet, small phase kicks
+    b = api.load_beam(nslice=S, npart=npart, nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+                      emitn=0.4e-6, beta=15, slicelength=LAMBDA0 * sample, current=3000.0,
+                      shotnoise=True, seed=99)
+    raw = np.zeros(S * 25)
+    L.orc_diag_beam_raw(S, ip(b.offsets), dp(b.x), dp(b.y), dp(b.px), dp(b.py), dp(b.gamma),
+                        dp(b.theta), 1, GAMMA0, dp(raw))
+    q, _ = api.finalize_beam(raw, 1, b.current, GAMMA0)
+    ne = 3000.0 * LAMBDA0 * sample / 4.8032e-11
+    ratio = np.mean(q[:, 25] ** 2) * ne
+    assert 0.8 < ratio < 1.2, ratio
+
+
+def test_field_diagnostics_of_a_gaussian_mode():
+    """Power of the normalised seed and rms size w0/2 of |u|^2 ~ exp(-2r^2/w0^2)."""
+    G, gm, w0, P = 101, 0.3e-3, 40e-6, 2.5e3
+    re, im = api.seed_gaussian(2, G, gm, LAMBDA0, P, w0)
+    of = OField(2, G, gm, LAMBDA0, LAMBDA0 * 3, 1, re, im)
+    raw = np.zeros(2 * 9)
+    L.orc_diag_field_raw(of.ptr, dp(raw))
+    q = np.zeros(2 * 9)
+    g = np.zeros(7)
+    L.orc_finalize_field(2, G, of.s.dgrid, LAMBDA0, LAMBDA0 * 3, dp(raw), dp(q), dp(g))
+    q = q.reshape(2, 9)
+    assert q[0, 0] == pytest.approx(P, rel=1e-12)
+    assert q[0, 2] == pytest.approx(w0 / 2, rel=1e-3)
+    assert q[0, 1] == pytest.approx(0.0, abs=1e-12)
+    assert g[0] == pytest.approx(2 * P * LAMBDA0 * 3 / 299792458.0, rel=1e-12)
+
+
+def test_finalize_beam_matches_library_finalize():
+    c = cases.make_case(nslice=3, npart=512)
+    b = c.beam
+    raw = np.zeros(3 * 27)
+    L.orc_diag_beam_raw(3, ip(b.offsets), dp(b.x), dp(b.y), dp(b.px), dp(b.py), dp(b.gamma),
+                        dp(b.theta), 2, GAMMA0, dp(raw))
+    q1 = np.zeros(3 * 29)
+    g1 = np.zeros(6)
+    L.orc_finalize_beam(3, 2, dp(raw), dp(b.current), GAMMA0, dp(q1), dp(g1))
+    q2, g2 = api.finalize_beam(raw, 2, b.current, GAMMA0)
+    assert np.allclose(q1.reshape(3, 29), q2, rtol=1e-14, atol=0)
+    assert np.allclose(g1, g2, rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# whole step: energy balance between beam and field
+# ---------------------------------------------------------------------------
+def _balance(div):
+    c = cases.make_case(nslice=1, npart=8192, ngrid=61, nsteps=40, seed_power=5e7, w0=40e-6,
+                        shotnoise=False, timerun=False)
+    steps = []
+    for s0 in c.steps:
+        for _ in range(div):
+            t = api.Step()
+            C.pointer(t)[0] = s0
+            t.delz = s0.delz / div
+            steps.append(t)
+    of = OField(1, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, c.seed_re, c.seed_im)
+    ob, braw, fraw = _oracle.run(c.beam, of, api.steps_array(steps), gamma0=GAMMA0,
+                                 lambda0=LAMBDA0, solver=0, timerun=False)
+    dgam = ob.gamma.mean() - c.beam.gamma.mean()
+    qf0, _ = api.finalize_field(fraw[0], of.s.dgrid, LAMBDA0, c.slicelength)
+    qf1, _ = api.finalize_field(fraw[-1], of.s.dgrid, LAMBDA0, c.slicelength)
+    return qf1[0, 0] - qf0[0, 0], -510999.06 * 3000.0 * dgam
+
+
+def test_energy_balance_beam_to_field():
+    """Steady state, seeded: the field power gained equals the beam power
+    lost, P_gain = -eev * I * <dgamma> (W).  This fixes the relative scale of
+    the deposition (source) and the gather (coupling).  The particle push sees
+    the field of the start of each step, so the balance is first order in
+    delz: halving the step halves the mismatch, and the Richardson
+    extrapolation balances to 0.5%."""
+    g1, l1 = _balance(1)
+    g2, l2 = _balance(2)
+    assert abs(g1) > 1e3  # there is an interaction
+    e1, e2 = g1 / l1 - 1, g2 / l2 - 1
+    assert abs(e1) < 0.1 and abs(e2) < 0.05
+    assert 1.7 < e1 / e2 < 2.3
+    assert abs(2 * e2 - e1) < 5e-3
