@@ -151,8 +151,8 @@ def main() -> int:
     ap.add_argument("--ngrid", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=192)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-slices", type=int, default=1536)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain field copies of 4096 x 256^2 FP32 cells (2 GiB "
                          "read + 2 GiB written each) to calibrate the HBM byte counters")
@@ -174,6 +174,8 @@ def main() -> int:
             for _ in range(3):
                 dst.copy_from(src)
             ctx.synchronize()
+            dst.close()
+            src.close()
             nbytes = a.slices * a.ngrid * a.ngrid * 8
             print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
                               "copies": 3}), flush=True)

@@ -12,6 +12,8 @@ namespace g4 {
 namespace {
 thread_local std::string g_last_error;
 }
+void context_child_added(Context* c);
+void context_child_removed(Context* c);
 
 void set_last_error(const std::string& m) { g_last_error = m; }
 
@@ -248,6 +250,25 @@ using namespace g4;
     return G4H_ERR_DEVICE;                                          \
   }
 
+static void release_context(g4h_context* ctx) {
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+namespace g4 {
+void context_child_added(Context* c) { ++c->children; }
+void context_child_removed(Context* c) {
+  if (--c->children == 0 && c->destroy_pending) release_context(static_cast<g4h_context*>(c));
+}
+}  // namespace g4
+
 extern "C" {
 
 int g4h_abi_version(void) { return 1; }
@@ -282,15 +303,11 @@ g4h_status g4h_context_create(int device, g4h_context** out) {
 
 void g4h_context_destroy(g4h_context* ctx) {
   if (!ctx) return;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  for (auto& p : ctx->pending) {
-    (void)hipEventDestroy(p.a);
-    (void)hipEventDestroy(p.b);
+  if (ctx->children > 0) {
+    ctx->destroy_pending = true;
+    return;
   }
-  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  release_context(ctx);
 }
 
 g4h_status g4h_context_synchronize(g4h_context* ctx) {
@@ -356,8 +373,9 @@ g4h_status g4h_beam_create(g4h_context* ctx, g4h_precision precision, int nslice
     throw;
   }
   *out = b;
+  context_child_added(ctx);
   if (g4h_status st = g4h_beam_set_current(b, b->current_h.data()); st != G4H_OK) {
-    delete b;
+    g4h_beam_destroy(b);
     *out = nullptr;
     return st;
   }
@@ -366,9 +384,11 @@ g4h_status g4h_beam_create(g4h_context* ctx, g4h_precision precision, int nslice
 
 void g4h_beam_destroy(g4h_beam* b) {
   if (!b) return;
-  (void)hipSetDevice(b->ctx->device);
-  (void)hipStreamSynchronize(b->ctx->stream);
+  g4::Context* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete b;
+  context_child_removed(ctx);
 }
 
 g4h_status g4h_beam_set_current(g4h_beam* b, const double* current) {
@@ -498,14 +518,17 @@ g4h_status g4h_field_create(g4h_context* ctx, g4h_precision precision, int nslic
     throw;
   }
   *out = f;
+  context_child_added(ctx);
   G4_API_END
 }
 
 void g4h_field_destroy(g4h_field* f) {
   if (!f) return;
-  (void)hipSetDevice(f->ctx->device);
-  (void)hipStreamSynchronize(f->ctx->stream);
+  g4::Context* ctx = f->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete f;
+  context_child_removed(ctx);
 }
 
 g4h_status g4h_field_upload(g4h_field* f, const double* re, const double* im, int first) {

@@ -84,6 +84,10 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t bytes = 0;
+  // Objects (beams, fields, runs) created from this context.  Destroying the
+  // context while some are alive defers the release until the last one goes.
+  int children = 0;
+  bool destroy_pending = false;
   // profiling
   bool profile = false;
   struct Prof {

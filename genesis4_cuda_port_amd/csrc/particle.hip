@@ -38,7 +38,6 @@ namespace {
 constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxHarm = 8;
-constexpr int kNSum = 13 + 2 * kMaxHarm;  // sums of the beam diagnostics
 constexpr int kNMinMax = 10;
 
 template <class P>
@@ -58,15 +57,20 @@ struct BeamPtr {
   double* th;
 };
 
-// theta reduced to [-pi, pi] in FP64, then FP32 sine and cosine.
-__device__ __forceinline__ void sincos_red(double a, float& s, float& c) {
+// MIXED sine and cosine of an FP64 phase: reduced to [-pi, pi] in FP64
+// (two-term 2*pi), then the hardware v_sin_f32 / v_cos_f32 (absolute error
+// ~1e-6, far inside the 1e-3 mixed-precision budget).
+__device__ __forceinline__ void fsincos(double a, float& s, float& c) {
   const double n = rint(a * 0.15915494309189535);
   double r = fma(-n, 6.283185307179586, a);
   r = fma(-n, 2.4492935982947064e-16, r);
-  sincosf(static_cast<float>(r), &s, &c);
+  const float rf = static_cast<float>(r);
+  s = __sinf(rf);
+  c = __cosf(rf);
 }
 
 // ---- transverse ---------------------------------------------------------
+// FP64: advance_focusing_plane (TrackBeamCUDA.cpp:27-59) verbatim.
 template <class P>
 __device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& coord, P& mom,
                                            P gz) {
@@ -95,6 +99,41 @@ __device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& co
   mom = a3 * xt * gz + a1 * mom;
 }
 
+// MIXED overload: fast reciprocal / square root and hardware sin/cos (the
+// focusing phase advance per half step is ~1e-2 rad).
+__device__ __forceinline__ void plane_step(int mode, float str, float off, float delz,
+                                           float& coord, float& mom, float gz) {
+  const float igz = __builtin_amdgcn_rcpf(gz);
+  if (mode == 0) {
+    coord += mom * delz * igz;
+    return;
+  }
+  if (mode == 1) {
+    const float foc = __builtin_amdgcn_sqrtf(str * igz);
+    const float omg = foc * delz;
+    const float a1 = __cosf(omg);
+    const float sn = __sinf(omg);
+    const float a2 = sn * __builtin_amdgcn_rcpf(foc);
+    const float a3 = -sn * foc;
+    const float xt = coord - off;
+    coord = a1 * xt + a2 * mom * igz + off;
+    mom = a3 * xt * gz + a1 * mom;
+    return;
+  }
+  const float foc = __builtin_amdgcn_sqrtf(-str * igz);
+  const float omg = foc * delz;
+  const float a1 = coshf(omg);
+  const float sh = sinhf(omg);
+  const float a2 = sh * __builtin_amdgcn_rcpf(foc);
+  const float a3 = sh * foc;
+  const float xt = coord - off;
+  coord = a1 * xt + a2 * mom * igz + off;
+  mom = a3 * xt * gz + a1 * mom;
+}
+
+__device__ __forceinline__ double psqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float psqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+
 // gamma^2 - 1 - extra (extra = aw^2 + px^2 + py^2 or px^2 + py^2)
 __device__ __forceinline__ double gz2(double g, double g0, double g2c, double aw2, bool with_aw) {
   (void)g0;
@@ -111,7 +150,7 @@ template <class P>
 __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g, P& x, P& y, P& px,
                                            P& py, bool first_half) {
   if (first_half && t.chicane) {
-    const P gz = sqrt(gz2(g, g0, t.g2chic, 0.0, false) - px * px - py * py);
+    const P gz = psqrt(gz2(g, g0, t.g2chic, 0.0, false) - px * px - py * py);
     P tmp = x;
     x = P(t.m00) * tmp + P(t.m01) * px / gz;
     px = P(t.m10) * tmp * gz + P(t.m11) * px;
@@ -123,9 +162,9 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
     px += P(t.cx);
     py += P(t.cy);
   }
-  const P gz = sqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
-  plane_step<P>(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
-  plane_step<P>(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
+  const P gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
+  plane_step(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
+  plane_step(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
 }
 
 // ---- particle -> cell (BeamSolverCUDA.cpp:99-116, FieldSolverADICUDA.cpp:
@@ -213,16 +252,16 @@ __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int 
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
     float s, c;
-    sincos_red(p.f[i].rharm * tthet, s, c);
+    fsincos(p.f[i].rharm * tthet, s, c);
     cre += rr[i] * c + ri[i] * s;
     cim += ri[i] * c - rr[i] * s;
   }
   const float db = db0 + m.ztemp1 * cre;
   const float d = dg * m.inv_g0;
-  const float opd = 1.0f + d;
-  const float eta = (db * m.inv_b0 - d * (2.0f + d)) / (opd * opd);
+  const float inv = __builtin_amdgcn_rcpf(1.0f + d);
+  const float eta = (db * m.inv_b0 - d * (2.0f + d)) * inv * inv;
   k2pp += -(m.c0 + eta * (m.k1 + m.k2 * eta));
-  k2gg += cim * m.inv_g0 / opd - ez;
+  k2gg += cim * m.inv_g0 * inv - ez;
 }
 
 // RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
@@ -305,8 +344,8 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k2gg - k3gg);
   t += double(stpz * (k2pp - k3pp));
-  k3gg /= 6.0f;
-  k3pp /= 6.0f;
+  k3gg *= (1.0f / 6.0f);
+  k3pp *= (1.0f / 6.0f);
   k2gg *= -0.5f;
   k2pp *= -0.5f;
   ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
@@ -318,8 +357,8 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   k2gg *= 2.0f;
   k2pp *= 2.0f;
   ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
-  g += stpz * (k3gg + k2gg / 6.0f);
-  t += double(stpz * (k3pp + k2pp / 6.0f));
+  g += stpz * (k3gg + k2gg * (1.0f / 6.0f));
+  t += double(stpz * (k3pp + k2pp * (1.0f / 6.0f)));
   dgamma = g;
   theta = t;
 }
@@ -351,19 +390,180 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-template <class P, int NF, bool DEP, bool DIAG>
+// Per-particle work of the pass, shared by the prefetching loop below.
+template <class P, int NF, bool DEP, int NH>
+struct PassState {
+  double acc[NH > 0 ? 13 + 2 * NH : 1];
+  P mm[NH > 0 ? 10 : 1];
+  int bminx = 0x7fffffff, bminy = 0x7fffffff, bmaxx = -1, bmaxy = -1;
+};
+
+template <class P, int NF, bool DEP, int NH>
+__device__ __forceinline__ void process(const ParticleParams& prm, const MixedK& mk, int s,
+                                        int64_t i, BeamPtr<P> b, P x, P y, P px, P py, P g,
+                                        double th, P scl, typename V2<P>::T* src, double2* win,
+                                        int wx0, int wy0, int ww, int wh, int* s_fb,
+                                        PassState<P, NF, DEP, NH>& ps) {
+  const int st = prm.stages;
+  const double g0 = prm.gamma0;
+  const int ng = prm.dep.ngrid;
+  if (st & ST_T1) transverse<P>(prm.t1, g0, g, x, y, px, py, true);
+  if (st & ST_PUSH) {
+    if constexpr (sizeof(P) == 8) {
+      double gg = g;
+      push64<NF>(prm.push, s, x, y, px, py, gg, th);
+      g = gg;
+    } else {
+      float gg = g;
+      push32<NF>(prm.push, mk, s, x, y, px, py, gg, th);
+      g = gg;
+    }
+  }
+  if (st & ST_R56) th += prm.r56 * gshift(g, g0);
+  if (st & ST_T2) transverse<P>(prm.t2, g0, g, x, y, px, py, false);
+  if (st & (ST_T1 | ST_T2)) {
+    b.x[i] = x;
+    b.y[i] = y;
+    b.px[i] = px;
+    b.py[i] = py;
+  }
+  if (st & ST_PUSH) b.g[i] = g;
+  if (st & (ST_PUSH | ST_R56)) b.th[i] = th;
+
+  // sine and cosine of the final phase, shared by the deposition (harm 1)
+  // and the h = 1 bunching sum
+  P sn1 = P(0), cs1 = P(1);
+  bool have1 = false;
+  auto phase1 = [&]() {
+    if (!have1) {
+      if constexpr (sizeof(P) == 8) {
+        double a, c;
+        sincos(th, &a, &c);
+        sn1 = a;
+        cs1 = c;
+      } else {
+        fsincos(th, sn1, cs1);
+      }
+      have1 = true;
+    }
+  };
+
+  if constexpr (DEP) {
+    if (scl != P(0)) {
+      const DepParams& d = prm.dep;
+      int ix, iy;
+      P wx, wy;
+      if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), ng, ix, iy, wx, wy)) {
+        const P dx = x - P(d.ax), dy = y - P(d.ay);
+        const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
+                       P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
+        P part, sn, cs;
+        if (d.harm == 1) {
+          phase1();
+          sn = sn1;
+          cs = cs1;
+        } else if constexpr (sizeof(P) == 8) {
+          sincos(static_cast<double>(d.harm) * th, &sn, &cs);
+        } else {
+          fsincos(static_cast<double>(d.harm) * th, sn, cs);
+        }
+        if constexpr (sizeof(P) == 8)
+          part = sqrt(faw2) * scl / g;
+        else
+          part = psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g);
+        const P cre = sn * part;
+        const P cim = cs * part;
+        const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
+        ps.bminx = min(ps.bminx, ix);
+        ps.bminy = min(ps.bminy, iy);
+        ps.bmaxx = max(ps.bmaxx, ix);
+        ps.bmaxy = max(ps.bmaxy, iy);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int cx = ix + (k & 1), cy = iy + (k >> 1);
+          const int lx = cx - wx0, ly = cy - wy0;
+          if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+            double* a = &win[lx + ly * ww].x;
+            lds_add(a, static_cast<double>(cre * w[k]));
+            lds_add(a + 1, static_cast<double>(cim * w[k]));
+          } else {
+            P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
+            glob_add(a, cre * w[k]);
+            glob_add(a + 1, cim * w[k]);
+            atomicMin(&s_fb[0], cx);
+            atomicMin(&s_fb[1], cy);
+            atomicMax(&s_fb[2], cx);
+            atomicMax(&s_fb[3], cy);
+          }
+        }
+      }
+    }
+  }
+
+  if constexpr (NH > 0) {
+    const double xv = x, yv = y, pxv = px, pyv = py;
+    const double dg = gshift(g, g0);
+    double* acc = ps.acc;
+    acc[0] += dg;
+    acc[1] += dg * dg;
+    acc[2] += xv;
+    acc[3] += xv * xv;
+    acc[4] += yv;
+    acc[5] += yv * yv;
+    acc[6] += pxv;
+    acc[7] += pxv * pxv;
+    acc[8] += pyv;
+    acc[9] += pyv * pyv;
+    acc[10] += xv * pxv;
+    acc[11] += yv * pyv;
+    acc[12] += 1.0;
+    // min/max in the storage type (exact); MIXED energy as gamma - gamma_ref
+    P* mm = ps.mm;
+    mm[0] = fmin(mm[0], x);
+    mm[1] = fmax(mm[1], x);
+    mm[2] = fmin(mm[2], px);
+    mm[3] = fmax(mm[3], px);
+    mm[4] = fmin(mm[4], y);
+    mm[5] = fmax(mm[5], y);
+    mm[6] = fmin(mm[6], py);
+    mm[7] = fmax(mm[7], py);
+    mm[8] = fmin(mm[8], g);
+    mm[9] = fmax(mm[9], g);
+    phase1();
+    acc[13] += cs1;
+    acc[14] += sn1;
+#pragma unroll
+    for (int h = 1; h < NH; ++h) {
+      if (h < prm.diag.nharm) {
+        if constexpr (sizeof(P) == 8) {
+          double sn, cs;
+          sincos(static_cast<double>(h + 1) * th, &sn, &cs);
+          acc[13 + 2 * h] += cs;
+          acc[14 + 2 * h] += sn;
+        } else {
+          float sn, cs;
+          fsincos(static_cast<double>(h + 1) * th, sn, cs);
+          acc[13 + 2 * h] += cs;
+          acc[14 + 2 * h] += sn;
+        }
+      }
+    }
+  }
+}
+
+template <class P, int NF, bool DEP, int NH>
 __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
                                                             ParticleParams prm) {
   using T2 = typename V2<P>::T;
+  constexpr int NS = NH > 0 ? 13 + 2 * NH : 1;
   extern __shared__ double2 win[];
   __shared__ int s_bb[4];   // particle bbox min x, min y, max x, max y (lower-left cells)
   __shared__ int s_fb[4];   // fallback (outside-window) touched-cell bbox
-  __shared__ double s_red[kWaves][kNSum + kNMinMax];
+  __shared__ double s_red[kWaves][NS + kNMinMax];
 
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
   const int64_t i0 = off[s], i1 = off[s + 1];
-  const int st = prm.stages;
   const double g0 = prm.gamma0;
 
   // ---- deposition set-up -------------------------------------------------
@@ -371,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   int wx0 = 0, wy0 = 0, ww = 0, wh = 0;
   P scl = P(0);
   T2* src = nullptr;
-  if (DEP) {
+  if constexpr (DEP) {
     src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
     scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
     const int4 bb = prm.dep.bbox[s];
@@ -416,24 +616,23 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     }
     __syncthreads();
   }
-  int bminx = 0x7fffffff, bminy = 0x7fffffff, bmaxx = -1, bmaxy = -1;
 
-  // ---- diagnostics accumulators -------------------------------------------
-  double acc[kNSum];
-  double mm[kNMinMax];
-  const int nharm = DIAG ? prm.diag.nharm : 0;
-  if (DIAG) {
+  PassState<P, NF, DEP, NH> ps;
+  if constexpr (NH > 0) {
 #pragma unroll
-    for (int k = 0; k < kNSum; ++k) acc[k] = 0.0;
-    mm[0] = 1.0e5;  mm[1] = -1.0e5;  // x
-    mm[2] = 1.0e5;  mm[3] = -1.0e5;  // px
-    mm[4] = 1.0e5;  mm[5] = -1.0e5;  // y
-    mm[6] = 1.0e5;  mm[7] = -1.0e5;  // py
-    mm[8] = 1.0e7;  mm[9] = 1.0;     // gamma
+    for (int k = 0; k < NS; ++k) ps.acc[k] = 0.0;
+    const P big = P(1.0e5);
+    ps.mm[0] = big;  ps.mm[1] = -big;  // x
+    ps.mm[2] = big;  ps.mm[3] = -big;  // px
+    ps.mm[4] = big;  ps.mm[5] = -big;  // y
+    ps.mm[6] = big;  ps.mm[7] = -big;  // py
+    // gamma: 1e7 and 1.0 (DiagnosticCUDA.cpp:222), shifted in MIXED
+    ps.mm[8] = sizeof(P) == 8 ? P(1.0e7) : P(1.0e7 - g0);
+    ps.mm[9] = sizeof(P) == 8 ? P(1.0) : P(1.0 - g0);
   }
 
   MixedK mk;
-  if (sizeof(P) == 4) {
+  if constexpr (sizeof(P) == 4) {
     mk.ztemp1 = float(-2.0 / prm.push.xks);
     mk.c0 = float(prm.push.c0);
     mk.k1 = float(prm.push.k1);
@@ -442,130 +641,44 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     mk.inv_g0 = float(1.0 / g0);
   }
 
-  for (int64_t i = i0 + tid; i < i1; i += kThreads) {
-    P x = b.x[i], y = b.y[i], px = b.px[i], py = b.py[i], g = b.g[i];
-    double th = b.th[i];
-    if (st & ST_T1) transverse<P>(prm.t1, g0, g, x, y, px, py, true);
-    if (st & ST_PUSH) {
-      if constexpr (sizeof(P) == 8) {
-        double gg = g;
-        push64<NF>(prm.push, s, x, y, px, py, gg, th);
-        g = gg;
-      } else {
-        float gg = g;
-        push32<NF>(prm.push, mk, s, x, y, px, py, gg, th);
-        g = gg;
-      }
+  // Software-pipelined particle loop: the next particle's state is loaded
+  // while the current one is processed.
+  int64_t i = i0 + tid;
+  bool have = i < i1;
+  P nx = P(0), ny = P(0), npx = P(0), npy = P(0), ngm = P(0);
+  double nth = 0.0;
+  if (have) {
+    nx = b.x[i];
+    ny = b.y[i];
+    npx = b.px[i];
+    npy = b.py[i];
+    ngm = b.g[i];
+    nth = b.th[i];
+  }
+  while (have) {
+    const P x = nx, y = ny, px = npx, py = npy, g = ngm;
+    const double th = nth;
+    const int64_t inext = i + kThreads;
+    const bool hnext = inext < i1;
+    if (hnext) {
+      nx = b.x[inext];
+      ny = b.y[inext];
+      npx = b.px[inext];
+      npy = b.py[inext];
+      ngm = b.g[inext];
+      nth = b.th[inext];
     }
-    if (st & ST_R56) th += prm.r56 * gshift(g, g0);
-    if (st & ST_T2) transverse<P>(prm.t2, g0, g, x, y, px, py, false);
-    if (st & (ST_T1 | ST_T2)) {
-      b.x[i] = x;
-      b.y[i] = y;
-      b.px[i] = px;
-      b.py[i] = py;
-    }
-    if (st & ST_PUSH) b.g[i] = g;
-    if (st & (ST_PUSH | ST_R56)) b.th[i] = th;
-
-    if (DEP && scl != P(0)) {
-      const DepParams& d = prm.dep;
-      int ix, iy;
-      P wx, wy;
-      {
-        if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), ng, ix, iy, wx, wy)) {
-          const P dx = x - P(d.ax), dy = y - P(d.ay);
-          const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
-                         P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
-          P part;
-          P sn, cs;
-          if constexpr (sizeof(P) == 8) {
-            part = sqrt(faw2) * scl / g;
-            sincos(static_cast<double>(d.harm) * th, &sn, &cs);
-          } else {
-            part = sqrt(faw2) * scl / (float(g0) + g);
-            sincos_red(static_cast<double>(d.harm) * th, sn, cs);
-          }
-          const P cre = sn * part;
-          const P cim = cs * part;
-          const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
-          bminx = min(bminx, ix);
-          bminy = min(bminy, iy);
-          bmaxx = max(bmaxx, ix);
-          bmaxy = max(bmaxy, iy);
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int cx = ix + (k & 1), cy = iy + (k >> 1);
-            const int lx = cx - wx0, ly = cy - wy0;
-            if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
-              double* a = &win[lx + ly * ww].x;
-              lds_add(a, static_cast<double>(cre * w[k]));
-              lds_add(a + 1, static_cast<double>(cim * w[k]));
-            } else {
-              P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
-              glob_add(a, cre * w[k]);
-              glob_add(a + 1, cim * w[k]);
-              atomicMin(&s_fb[0], cx);
-              atomicMin(&s_fb[1], cy);
-              atomicMax(&s_fb[2], cx);
-              atomicMax(&s_fb[3], cy);
-            }
-          }
-        }
-      }
-    }
-
-    if (DIAG) {
-      const double xv = x, yv = y, pxv = px, pyv = py;
-      const double dg = gshift(g, g0);
-      const double ga = gabs(g, g0);
-      acc[0] += dg;
-      acc[1] += dg * dg;
-      acc[2] += xv;
-      acc[3] += xv * xv;
-      acc[4] += yv;
-      acc[5] += yv * yv;
-      acc[6] += pxv;
-      acc[7] += pxv * pxv;
-      acc[8] += pyv;
-      acc[9] += pyv * pyv;
-      acc[10] += xv * pxv;
-      acc[11] += yv * pyv;
-      acc[12] += 1.0;
-      mm[0] = fmin(mm[0], xv);
-      mm[1] = fmax(mm[1], xv);
-      mm[2] = fmin(mm[2], pxv);
-      mm[3] = fmax(mm[3], pxv);
-      mm[4] = fmin(mm[4], yv);
-      mm[5] = fmax(mm[5], yv);
-      mm[6] = fmin(mm[6], pyv);
-      mm[7] = fmax(mm[7], pyv);
-      mm[8] = fmin(mm[8], ga);
-      mm[9] = fmax(mm[9], ga);
-#pragma unroll
-      for (int h = 0; h < kMaxHarm; ++h) {
-        if (h < nharm) {
-          if constexpr (sizeof(P) == 8) {
-            double sn, cs;
-            sincos(static_cast<double>(h + 1) * th, &sn, &cs);
-            acc[13 + 2 * h] += cs;
-            acc[14 + 2 * h] += sn;
-          } else {
-            float sn, cs;
-            sincos_red(static_cast<double>(h + 1) * th, sn, cs);
-            acc[13 + 2 * h] += cs;
-            acc[14 + 2 * h] += sn;
-          }
-        }
-      }
-    }
+    process<P, NF, DEP, NH>(prm, mk, s, i, b, x, y, px, py, g, th, scl, src, win, wx0, wy0, ww,
+                            wh, s_fb, ps);
+    i = inext;
+    have = hnext;
   }
 
-  if (DEP) {
-    atomicMin(&s_bb[0], bminx);
-    atomicMin(&s_bb[1], bminy);
-    atomicMax(&s_bb[2], bmaxx);
-    atomicMax(&s_bb[3], bmaxy);
+  if constexpr (DEP) {
+    atomicMin(&s_bb[0], ps.bminx);
+    atomicMin(&s_bb[1], ps.bminy);
+    atomicMax(&s_bb[2], ps.bmaxx);
+    atomicMax(&s_bb[3], ps.bmaxy);
     __syncthreads();
     for (int k = tid; k < ww * wh; k += kThreads) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
@@ -594,20 +707,22 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     }
   }
 
-  if (DIAG) {
+  if constexpr (NH > 0) {
+    const int nharm = prm.diag.nharm;
     const int lane = tid & 63, wave = tid >> 6;
     const int nsum = 13 + 2 * nharm;
 #pragma unroll
-    for (int k = 0; k < kNSum; ++k) {
+    for (int k = 0; k < NS; ++k) {
       if (k < nsum) {
-        const double v = wave_sum(acc[k]);
+        const double v = wave_sum(ps.acc[k]);
         if (lane == 0) s_red[wave][k] = v;
       }
     }
 #pragma unroll
     for (int k = 0; k < kNMinMax; ++k) {
-      const double v = (k & 1) ? wave_max(mm[k]) : wave_min(mm[k]);
-      if (lane == 0) s_red[wave][kNSum + k] = v;
+      const double m = static_cast<double>(ps.mm[k]);
+      const double v = (k & 1) ? wave_max(m) : wave_min(m);
+      if (lane == 0) s_red[wave][NS + k] = v;
     }
     __syncthreads();
     double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
@@ -619,10 +734,11 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       out[dst] = v;
     } else if (tid >= 64 && tid < 64 + kNMinMax) {
       const int k = tid - 64;
-      double v = s_red[0][kNSum + k];
+      double v = s_red[0][NS + k];
       for (int w = 1; w < kWaves; ++w)
-        v = (k & 1) ? fmax(v, s_red[w][kNSum + k]) : fmin(v, s_red[w][kNSum + k]);
-      // mm order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
+        v = (k & 1) ? fmax(v, s_red[w][NS + k]) : fmin(v, s_red[w][NS + k]);
+      if (sizeof(P) == 4 && k >= 8) v += g0;  // MIXED energy extremes were shifted
+      // order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
       out[G4H_B_XMIN + k] = v;
     }
   }
@@ -638,30 +754,35 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
   cell[i] = cell_of<P>(b.x[i], b.y[i], P(gm), P(dg), ng, ix, iy, wx, wy) ? ix + iy * ng : -1;
 }
 
-template <class P, int NF, bool DEP, bool DIAG>
+template <class P, int NF, bool DEP, int NH>
 void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
   const size_t lds = DEP ? static_cast<size_t>(p.dep.window_cap) * sizeof(double2) : 0;
   ctx->launch(G4H_K_PARTICLE, [&] {
-    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, DIAG>), dim3(b->nslice), dim3(kThreads), lds,
+    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH>), dim3(b->nslice), dim3(kThreads), lds,
                        ctx->stream, bp, b->off.get(), p);
   });
+}
+
+template <class P, int NF, bool DEP>
+void dispatch_nh(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
+  if (!(p.stages & ST_DIAG)) return launch_t<P, NF, DEP, 0>(ctx, b, p, bp);
+  const int nh = p.diag.nharm;
+  if (nh == 1) return launch_t<P, NF, DEP, 1>(ctx, b, p, bp);
+  if (nh == 2) return launch_t<P, NF, DEP, 2>(ctx, b, p, bp);
+  if (nh <= 4) return launch_t<P, NF, DEP, 4>(ctx, b, p, bp);
+  return launch_t<P, NF, DEP, 8>(ctx, b, p, bp);
 }
 
 template <class P>
 void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
   const bool dep = (p.stages & ST_DEP) != 0;
-  const bool diag = (p.stages & ST_DIAG) != 0;
   const bool single = !(p.stages & ST_PUSH) || p.push.nfield == 1;
   if (single) {
-    if (dep && diag) return launch_t<P, 1, true, true>(ctx, b, p, bp);
-    if (dep) return launch_t<P, 1, true, false>(ctx, b, p, bp);
-    if (diag) return launch_t<P, 1, false, true>(ctx, b, p, bp);
-    return launch_t<P, 1, false, false>(ctx, b, p, bp);
+    if (dep) return dispatch_nh<P, 1, true>(ctx, b, p, bp);
+    return dispatch_nh<P, 1, false>(ctx, b, p, bp);
   }
-  if (dep && diag) return launch_t<P, 0, true, true>(ctx, b, p, bp);
-  if (dep) return launch_t<P, 0, true, false>(ctx, b, p, bp);
-  if (diag) return launch_t<P, 0, false, true>(ctx, b, p, bp);
-  return launch_t<P, 0, false, false>(ctx, b, p, bp);
+  if (dep) return dispatch_nh<P, 0, true>(ctx, b, p, bp);
+  return dispatch_nh<P, 0, false>(ctx, b, p, bp);
 }
 
 }  // namespace

@@ -20,6 +20,8 @@
 
 namespace g4 {
 int slippage_host(Field* f, double slippage, double sample, int timerun, int periodic);
+void context_child_added(Context* c);
+void context_child_removed(Context* c);
 }
 
 using namespace g4;
@@ -132,14 +134,17 @@ g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,
     throw;
   }
   *out = r;
+  context_child_added(ctx);
   G4_API_END
 }
 
 void g4h_run_destroy(g4h_run* r) {
   if (!r) return;
-  (void)hipSetDevice(r->ctx->device);
-  (void)hipStreamSynchronize(r->ctx->stream);
+  g4::Context* ctx = r->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete r;
+  context_child_removed(ctx);
 }
 
 g4h_status g4h_run_advance(g4h_run* r, int nsteps, int* done) {

@@ -43,7 +43,9 @@ int g4h_device_count(void);
 /* Message of the last failed call on this host thread ("" if none). */
 const char* g4h_last_error(void);
 
-/* Create a context on `device` with its own stream (default priority). */
+/* Create a context on `device` with its own stream (default priority).
+ * Destroying a context while beams, fields or runs created from it are alive
+ * defers the release until the last of them is destroyed. */
 g4h_status g4h_context_create(int device, g4h_context** out);
 void g4h_context_destroy(g4h_context* ctx);
 /* Wait for all work enqueued on the context's stream. */

@@ -7,6 +7,33 @@
  *
  * Every function names the reference file:line it follows.  All arrays are
  * caller-owned host memory in FP64.
+ *
+ * How each function is pinned (tests/test_oracle_pins.py), since the
+ * reference's CPU path (upstream GENESIS 1.3 v4.6.14) is not on disk and its
+ * CUDA path cannot be built here, and the reference holds no golden vectors:
+ *   orc_fc                 Bessel closed form vs scipy.special.jv
+ *   orc_track_transverse   quad map vs scipy solve_ivp; natural focusing
+ *                          wavenumber aw*ku/gamma; det = 1 (symplectic);
+ *                          chicane -> identity at zero angle
+ *   orc_apply_r56          small-angle R56 = 2 theta^2 (ld + 2 lb / 3)
+ *   orc_beam_advance       zero-field closed form; resonant particle at rest;
+ *                          4th-order convergence of RK4 in a constant field
+ *   orc_deposit            single-particle known answers; partition of unity
+ *   orc_cell_index         brute-force floor rule incl. grid-edge inputs
+ *   orc_adi_*              Thomas vs dense complex solve (numpy); factors
+ *                          reproduce the tridiagonal matrix; power
+ *                          conservation; Gaussian-beam diffraction
+ *   orc_fft_advance, orc_dft  numpy FFT; 2S identity; odd grids equal the
+ *                          textbook Fresnel propagator and numpy-layout filter;
+ *                          Gaussian-beam diffraction; ADI ~ FFT on a smooth beam
+ *   orc_apply_slippage     integer ring model
+ *   orc_diag_*, orc_finalize_*  quiet-start moments and zero bunching;
+ *                          shot-noise statistics; Gaussian-mode power, size
+ *   orc_run                beam-to-field energy balance (first order in delz)
+ * Parity unpinned: the even-grid half-bin offset of the FFT k2 / sigmoid
+ * tables follows FieldSolverFFTCUDA.cpp:297-316 and nothing independent fixes
+ * it; the physical constants (vacimp, eev, ce) and fc(h) follow upstream
+ * definitions that are absent from the reference.
  */
 #ifndef G4ORACLE_H_
 #define G4ORACLE_H_

@@ -186,8 +186,47 @@ def test_field_track_fp64(gpu_ctx, kind, ngrid):
         assert e < 1e-12
 
 
-def test_fft_filter_fp64(gpu_ctx):
-    c = cases.make_case(nslice=3, npart=1024, ngrid=32, seed_power=1e6)
+@pytest.mark.parametrize("prec,ngrid", [(api.FP64, g) for g in (4, 8, 16, 64, 128, 256, 512, 1024,
+                                                                   48, 96)]
+                         + [(api.MIXED, g) for g in (16, 64, 256, 1024)])
+def test_fft_sizes(gpu_ctx, prec, ngrid):
+    """Spectral step at every transform shape: one or several lines per wave
+    (G < 64), one to sixteen points per lane (G >= 64), and the direct DFT
+    of sizes that are not powers of two."""
+    print(f"case fft_sizes prec={prec} ngrid={ngrid}", flush=True)
+    c = cases.make_case(nslice=3, npart=256, ngrid=ngrid, seed_power=1e6)
+    st = quad_step()
+    re, im = noisy_field(c, scale=0.1)
+    f = gpu_field(gpu_ctx, c, prec, kind=api.FFT, re=re, im=im, first=2)
+    b = gpu_beam(gpu_ctx, c, prec)
+    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, re, im, first=2)
+    o = c.beam
+    tol = 1e-12 if prec == api.FP64 else 2e-5
+    for it in range(2):
+        f.track(b, st, st.delz)
+        sre = np.zeros(c.nslice * c.ngrid * c.ngrid)
+        sim = np.zeros_like(sre)
+        _oracle.lib().orc_deposit(c.nslice, ip(o.offsets), dp(o.x), dp(o.y), dp(o.gamma),
+                                  dp(o.theta), of.ptr, st, dp(o.current), st.delz, dp(sre),
+                                  dp(sim))
+        _oracle.lib().orc_fft_advance(of.ptr, dp(sre), dp(sim), st.delz, 0, 1.0, 1.0, 1.0)
+        gre, gim, first = f.download()
+        assert first == 2
+        e = relerr(gre + 1j * gim, of.re + 1j * of.im)
+        print(f"  step {it}: rel err {e:.3e}", flush=True)
+        assert e < tol
+    fraw = f.diag_raw()
+    ofraw = np.zeros_like(fraw)
+    _oracle.lib().orc_diag_field_raw(of.ptr, dp(ofraw))
+    # field sums weight the far cells by dx^2, where the round-off of two
+    # different FFT factorisations is largest relative to the field: the FP64
+    # bar of 1e-10 applies here, not the 1e-12 of the field itself
+    assert relerr(fraw, ofraw) < (1e-10 if prec == api.FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("ngrid", [32, 128])
+def test_fft_filter_fp64(gpu_ctx, ngrid):
+    c = cases.make_case(nslice=3, npart=1024, ngrid=ngrid, seed_power=1e6)
     st = quad_step()
     f = gpu_field(gpu_ctx, c, kind=api.FFT)
     f.init_solver(api.FFT, source_filter=True, xcut=0.3, ycut=0.4, sigmoid=0.1)

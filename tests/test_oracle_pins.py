@@ -399,6 +399,42 @@ def test_fft_step_source_is_added_twice_and_identity_at_zero_length():
     assert np.allclose(z.re + 1j * z.im, u, atol=1e-13)
 
 
+@pytest.mark.parametrize("G", [15, 33])
+def test_fft_propagator_and_filter_match_textbook_layout_on_odd_grids(G):
+    """For odd G the shifted storage index of the reference's k2 / sigmoid
+    tables is the standard signed-frequency layout (numpy fftfreq), so the
+    step must equal the textbook Fresnel propagator exp(-i k_perp^2 dz/(2 ks))
+    with physical wavenumbers 2 pi fftfreq(G, dgrid), and the filtered source
+    must be 2 ifft2(sigma(k) fft2(S)).  Different cut-offs in x and y pin the
+    orientation (cell index ix + iy*G)."""
+    gm = 2e-4
+    dgrid = 2 * gm / (G - 1)
+    ks = 2 * np.pi / LAMBDA0
+    rng = np.random.default_rng(G)
+    u = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+    s = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+    dz = 0.7
+    k = 2 * np.pi * np.fft.fftfreq(G, d=dgrid)
+    prop = np.exp(-1j * (k[None, :] ** 2 + k[:, None] ** 2) * dz / (2 * ks))
+    a = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, u.real.ravel(), u.imag.ravel())
+    L.orc_fft_advance(a.ptr, None, None, dz, 0, 1.0, 1.0, 1.0)
+    want = np.fft.ifft2(np.fft.fft2(u) * prop)
+    assert np.allclose((a.re + 1j * a.im).reshape(G, G), want, rtol=0, atol=1e-12)
+    xc, yc, sig = 0.2, 0.35, 0.05
+    fx = np.fft.fftfreq(G) / xc
+    fy = np.fft.fftfreq(G) / yc
+    r = np.sqrt(fx[None, :] ** 2 + fy[:, None] ** 2)
+    filt = 1.0 / (1.0 + np.exp((r - 1.0) / sig))
+    z = np.zeros(G * G)
+    b = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, z, z)
+    L.orc_fft_advance(b.ptr, dp(np.ascontiguousarray(s.real.ravel())),
+                      dp(np.ascontiguousarray(s.imag.ravel())), dz, 1, xc, yc, sig)
+    want = 2 * np.fft.ifft2(filt * np.fft.fft2(s))
+    assert np.allclose((b.re + 1j * b.im).reshape(G, G), want, rtol=0, atol=1e-12)
+    # the filter is not the identity here: it removes part of the source
+    assert np.linalg.norm(want - 2 * s) > 0.1 * np.linalg.norm(2 * s)
+
+
 def _gaussian_rms(of, G, gm):
     p = (of.re ** 2 + of.im ** 2).reshape(G, G)
     xs = np.linspace(-gm, gm, G)
