@@ -140,6 +140,12 @@ PROTOTYPES = {
     "g4h_lattice_fodo": (C.c_int, [C.c_double, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_double, C.c_int, C.POINTER(Step), C.c_int,
                                    C.POINTER(C.c_int)]),
+    "g4h_lattice_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, C.c_double, C.POINTER(Step), C.c_int,
+                                    C.POINTER(C.c_int)]),
+    "g4h_lattice_match": (C.c_int, [C.POINTER(Step), C.c_int, C.c_double, C.c_double, _dp]),
+    "g4h_deck_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "g4h_deck_run": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
 }
 
 

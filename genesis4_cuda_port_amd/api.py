@@ -303,6 +303,42 @@ def lattice_fodo(lambdau: float, aw_mod, nper: int, quad_spacing: int, quad_len:
     return arr
 
 
+def lattice_parse(text: str, beamline: str, delz: float, lambda0: float, gamma0: float,
+                  output_step: int = 1, zstop: float = 1e9):
+    """Per-step table of `beamline` in lattice file text (g4h_lattice_parse)."""
+    n = C.c_int()
+    args = (text.encode(), beamline.encode(), delz, lambda0, gamma0, output_step, zstop)
+    check(lib().g4h_lattice_parse(*args, None, 0, C.byref(n)))
+    arr = (Step * n.value)()
+    check(lib().g4h_lattice_parse(*args, arr, n.value, C.byref(n)))
+    return arr
+
+
+def lattice_match(steps, gamma0: float, zmatch: float):
+    """Matched (betax, alphax, betay, alphay) of the section [0, zmatch]."""
+    t = np.empty(4)
+    check(lib().g4h_lattice_match(steps, len(steps), gamma0, zmatch, _dp(t)))
+    return tuple(float(v) for v in t)
+
+
+def deck_parse(text: str):
+    """Namelists of an input deck, in file order: [(name, [(key, value), ...]), ...]."""
+    import json
+    need = C.c_int()
+    check(lib().g4h_deck_parse(text.encode(), None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    check(lib().g4h_deck_parse(text.encode(), buf, need.value, C.byref(need)))
+    return [(d["name"], [tuple(kv) for kv in d["keys"]]) for d in json.loads(buf.value.decode())]
+
+
+def deck_run(deck_path: str, out_dir: str, device: int = 0) -> int:
+    """Run an input deck on the GPU; returns the number of records written."""
+    n = C.c_int()
+    check(lib().g4h_deck_run(str(deck_path).encode(), str(out_dir).encode(), device,
+                             C.byref(n)))
+    return int(n.value)
+
+
 @dataclass
 class BeamArrays:
     x: np.ndarray
@@ -321,9 +357,14 @@ class BeamArrays:
 
 def load_beam(*, nslice: int, npart: int, nbins: int, gamma0: float, espread_rel: float,
               emitn: float, beta: float, alpha: float = 0.0, slicelength: float, current,
-              shotnoise: bool, seed: int, chirp_rel: float = 0.0) -> BeamArrays:
+              shotnoise: bool, seed: int, chirp_rel: float = 0.0, emitn_y: float | None = None,
+              beta_y: float | None = None, alpha_y: float | None = None) -> BeamArrays:
+    """Quiet-start beam (g4h_load_beam); the y plane defaults to the x plane."""
     cur = _f64(np.broadcast_to(np.asarray(current, dtype=np.float64), (nslice,)).copy(), nslice)
-    spec = BeamSpec(gamma0, espread_rel, chirp_rel, emitn, emitn, beta, beta, alpha, alpha,
+    ey = emitn if emitn_y is None else emitn_y
+    by = beta if beta_y is None else beta_y
+    ay = alpha if alpha_y is None else alpha_y
+    spec = BeamSpec(gamma0, espread_rel, chirp_rel, emitn, ey, beta, by, alpha, ay,
                     0.0, 0.0, 0.0, 0.0, slicelength, nslice, npart, nbins, int(shotnoise), seed)
     n = nslice * npart
     arrs = [np.empty(n) for _ in range(6)]

@@ -30,6 +30,8 @@ SOURCES = [
     "abi.cpp",
     "run.cpp",
     "frontend.cpp",
+    "deck.cpp",
+    "deck_run.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -78,7 +80,23 @@ def build(clean: bool = False, jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    _build_cli()
     return LIB
+
+
+CLI = OUT / "g4run"
+
+
+def _build_cli() -> Path:
+    """The g4run deck runner, linked against libg4hip.so beside it."""
+    tmp = CLI.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", str(CSRC / "g4run_main.cpp"), "-o", str(tmp),
+           f"-L{OUT}", "-lg4hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g4run link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def build_oracle() -> Path:

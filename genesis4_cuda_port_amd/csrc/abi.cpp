@@ -153,6 +153,7 @@ void make_push(PushParams& p, const g4h_step& st, double gamma0, double delz,
     g.data = f->data();
     g.gridmax = f->gridmax;
     g.dgrid = f->dgrid;
+    g.inv_dgrid = 1.0 / f->dgrid;
     g.rtmp = g4h_undulator_fc(st.aw, st.helical, f->harm) / f->xks;
     g.rharm = static_cast<double>(f->harm);
     g.first = f->first;
@@ -197,6 +198,7 @@ bool make_dep(DepParams& d, Field* f, Beam* b, const g4h_step& st, double delz) 
   d.scl_factor = fc * G4H_VACIMP * f->xks * delz / (4.0 * G4H_EEV * f->dgrid * f->dgrid);
   d.gridmax = f->gridmax;
   d.dgrid = f->dgrid;
+  d.inv_dgrid = 1.0 / f->dgrid;
   d.ax = st.ax;
   d.ay = st.ay;
   d.kx = st.kx;
@@ -205,7 +207,7 @@ bool make_dep(DepParams& d, Field* f, Beam* b, const g4h_step& st, double delz) 
   d.grady = st.grady;
   d.ngrid = f->ngrid;
   d.harm = f->harm;
-  d.window_cap = 3840;  // 60 KiB of FP64 complex accumulators
+  d.window_cap = 3072;  // 48 KiB of FP64 complex accumulators: 3 workgroups per CU
   return true;
 }
 

@@ -19,6 +19,7 @@
 // use a direct DFT in LDS.  With the source filter on (non-separable), the filtered
 // source T = 2 IFFT2(filt FFT2(S))/G^2 is built in a work plane first
 // (FWD rows, FWD*filt*INV columns, INV rows) and added in place of 2S.
+#include "fft_regs.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -288,240 +289,137 @@ __global__ __launch_bounds__(512) void fft_cols_dft_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Register-resident transforms for power-of-two G (2..1024).
-//
-// A line of G points is spread over L = G/R lanes of a wave, R = max(1, G/64)
-// points per lane: lane t holds points e = t + L*r, r < R.  Butterfly spans
-// s >= L pair registers of one lane; spans s < L pair lanes t and t^s and cost
-// one cross-lane shuffle per value.  The forward transform is decimation in
-// frequency (natural order in, bit-reversed out) and the inverse decimation
-// in time (bit-reversed in, natural out), so FFT -> multiply -> IFFT needs no
-// reordering: the multiplier is applied in bit-reversed order.  No LDS and no
-// barrier is needed for the transform itself.
+// Power-of-two sizes: register-resident four-step transforms (fft_regs.hpp).
 // ---------------------------------------------------------------------------
-template <class T2>
-__device__ __forceinline__ T2 shfl_xor2(T2 v, int m) {
-  T2 r;
-  r.x = __shfl_xor(v.x, m, 64);
-  r.y = __shfl_xor(v.y, m, 64);
-  return r;
-}
-
-template <int G>
-struct RegShape {
-  static constexpr int R = G >= 64 ? G / 64 : 1;  // points per lane
-  static constexpr int L = G / R;                 // lanes per line
-  static constexpr int LPW = 64 / L;              // lines per wave
+// Waves per workgroup: as many as fit 48 KiB of transpose buffers, at most 4.
+template <class P, int G>
+struct RegCfg {
+  using S = fftr::Shape<G>;
+  static constexpr int kWaveBytes = S::LPW * S::BUF * 2 * static_cast<int>(sizeof(P));
+  static constexpr int kWaves =
+      kWaveBytes * 4 <= 48 * 1024 ? 4 : (kWaveBytes * 2 <= 48 * 1024 ? 2 : 1);
+  static constexpr int kThreads = 64 * kWaves;
+  // column tile: one column per line slot of the workgroup
+  static constexpr int kTile = kWaves * S::LPW < G ? kWaves * S::LPW : G;
 };
 
-// In-register butterflies of span s = RS*L (pairs r, r+RS of one lane).
-template <class T2, int G, int RS, bool INV>
-__device__ __forceinline__ void reg_stage(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  constexpr int R = RegShape<G>::R, L = RegShape<G>::L;
-  constexpr int s = RS * L;
-  constexpr int tstep = G / (2 * s);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if ((r & RS) == 0) {
-      const int k = (t + L * r) & (2 * s - 1);
-      if (!INV) {
-        const T2 a = v[r], b = v[r + RS];
-        v[r] = cadd(a, b);
-        v[r + RS] = cmul(csub(a, b), tw[k * tstep]);
-      } else {
-        const T2 a = v[r], b = cmul(v[r + RS], conjv(tw[k * tstep]));
-        v[r] = cadd(a, b);
-        v[r + RS] = csub(a, b);
-      }
-    }
-  }
-}
-
-// Cross-lane butterflies of span S < L (lanes t and t^S).
-template <class T2, int G, int S, bool INV>
-__device__ __forceinline__ void lane_stage(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  constexpr int R = RegShape<G>::R;
-  constexpr int tstep = G / (2 * S);
-  const bool lower = (t & S) == 0;
-  T2 w = tw[(t & (S - 1)) * tstep];
-  if (INV) w = conjv(w);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (!INV) {
-      const T2 q = shfl_xor2(v[r], S);
-      v[r] = lower ? cadd(v[r], q) : cmul(csub(q, v[r]), w);
-    } else {
-      const T2 tv = lower ? v[r] : cmul(v[r], w);
-      const T2 q = shfl_xor2(tv, S);
-      v[r] = lower ? cadd(tv, q) : csub(q, tv);
-    }
-  }
-}
-
-template <class T2, int G, int RS>
-__device__ __forceinline__ void reg_stages_fwd(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  if constexpr (RS >= 1) {
-    reg_stage<T2, G, RS, false>(v, t, tw);
-    reg_stages_fwd<T2, G, RS / 2>(v, t, tw);
-  }
-}
-template <class T2, int G, int RS>
-__device__ __forceinline__ void reg_stages_inv(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  if constexpr (RS >= 1 && RS < RegShape<G>::R) {
-    reg_stage<T2, G, RS, true>(v, t, tw);
-    reg_stages_inv<T2, G, RS * 2>(v, t, tw);
-  }
-}
-template <class T2, int G, int S>
-__device__ __forceinline__ void lane_stages_fwd(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  if constexpr (S >= 1) {
-    lane_stage<T2, G, S, false>(v, t, tw);
-    lane_stages_fwd<T2, G, S / 2>(v, t, tw);
-  }
-}
-template <class T2, int G, int S>
-__device__ __forceinline__ void lane_stages_inv(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  if constexpr (S >= 1 && S < RegShape<G>::L) {
-    lane_stage<T2, G, S, true>(v, t, tw);
-    lane_stages_inv<T2, G, S * 2>(v, t, tw);
-  }
-}
-
-// Forward DIF, spans G/2 .. 1: natural order in, bit-reversed order out.
-template <class T2, int G>
-__device__ __forceinline__ void reg_fft_fwd(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  reg_stages_fwd<T2, G, RegShape<G>::R / 2>(v, t, tw);
-  lane_stages_fwd<T2, G, RegShape<G>::L / 2>(v, t, tw);
-}
-
-// Inverse DIT with conjugate twiddles, spans 1 .. G/2 (unscaled):
-// bit-reversed order in, natural order out.
-template <class T2, int G>
-__device__ __forceinline__ void reg_fft_inv(T2 (&v)[RegShape<G>::R], int t, const T2* tw) {
-  lane_stages_inv<T2, G, 1>(v, t, tw);
-  reg_stages_inv<T2, G, 1>(v, t, tw);
-}
-
-// Loads G/2 twiddles and, when wanted, the G bit-reversed multipliers to LDS.
 template <class T2>
-__device__ __forceinline__ void load_tables(T2* tw, T2* mu, int G, const double2* tw_d,
-                                            const double2* mult_d, bool want_mult) {
-  for (int k = threadIdx.x; k < G / 2; k += blockDim.x) {
-    tw[k].x = static_cast<decltype(tw[k].x)>(tw_d[k].x);
-    tw[k].y = static_cast<decltype(tw[k].x)>(tw_d[k].y);
+__device__ __forceinline__ void load_table(T2* dst, const double2* src, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    dst[k].x = static_cast<decltype(dst[k].x)>(src[k].x);
+    dst[k].y = static_cast<decltype(dst[k].x)>(src[k].y);
   }
-  if (want_mult)
-    for (int k = threadIdx.x; k < G; k += blockDim.x) {
-      mu[k].x = static_cast<decltype(tw[k].x)>(mult_d[k].x);
-      mu[k].y = static_cast<decltype(tw[k].x)>(mult_d[k].y);
-    }
 }
 
-// Row pass: each wave transforms LPW whole rows; waves stride over the rows.
+// Row pass: each wave transforms LPW rows at a time; waves stride over rows.
 template <class P, int G>
-__global__ __launch_bounds__(256) void fft_rows_reg_kernel(typename C2<P>::T* u, int64_t nrows,
-                                                           const double2* tw_d,
-                                                           const double2* mult_d, int mode,
-                                                           double scale) {
+__global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
+    typename C2<P>::T* u, int64_t nrows, const double2* tw_d, const double2* mult_d, int mode,
+    double scale) {
   using T2 = typename C2<P>::T;
-  constexpr int R = RegShape<G>::R, L = RegShape<G>::L, LPW = RegShape<G>::LPW;
-  __shared__ T2 tw[G / 2 > 0 ? G / 2 : 1];
+  using S = fftr::Shape<G>;
+  using C = RegCfg<P, G>;
+  constexpr int R = S::R, L = S::L;
+  __shared__ T2 tw[G];
   __shared__ T2 mu[G];
-  load_tables(tw, mu, G, tw_d, mult_d, mode == ROW_PROP);
+  __shared__ T2 buf[C::kWaves * S::LPW * S::BUF];
+  load_table(tw, tw_d, G);
+  if (mode == ROW_PROP) load_table(mu, mult_d, G);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int t = lane % L;
-  const int64_t wave = blockIdx.x * static_cast<int64_t>(blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L, t = lane % L;
+  T2* lbuf = buf + (wave * S::LPW + g) * S::BUF;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * C::kWaves;
   const P sc = static_cast<P>(scale);
-  for (int64_t grp = wave; grp * LPW < nrows; grp += nwaves) {
-    const int64_t line = grp * LPW + lane / L;
-    if (line >= nrows) continue;  // whole lane groups drop out together
-    T2* g = u + line * G;
+  for (int64_t grp = blockIdx.x * static_cast<int64_t>(C::kWaves) + wave; grp * S::LPW < nrows;
+       grp += nwaves) {
+    const int64_t line = grp * S::LPW + g;
+    const bool valid = line < nrows;  // invalid line groups compute on zeros
+    T2* row = u + line * G;
     T2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = g[t + L * r];
-    if (mode == ROW_PROP || mode == ROW_FWD) reg_fft_fwd<T2, G>(v, t, tw);
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      v[r].x = 0;
+      v[r].y = 0;
+      if (valid) v[r] = row[t + L * r];
+    });
+    if (mode == ROW_PROP || mode == ROW_FWD) fftr::forward<T2, G>(v, t, tw, lbuf);
     if (mode == ROW_PROP) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = cmul(v[r], mu[t + L * r]);
+      fftr::static_for<0, R>([&](auto ic) {
+        constexpr int q = decltype(ic)::value;
+        v[q] = fftr::cmul(v[q], mu[q * L + t]);
+      });
     }
-    if (mode == ROW_PROP || mode == ROW_INV) reg_fft_inv<T2, G>(v, t, tw);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      T2 o = v[r];
-      o.x *= sc;
-      o.y *= sc;
-      g[t + L * r] = o;
+    if (mode == ROW_PROP || mode == ROW_INV) fftr::inverse<T2, G>(v, t, tw, lbuf);
+    if (valid) {
+      fftr::static_for<0, R>([&](auto ic) {
+        constexpr int r = decltype(ic)::value;
+        T2 o = v[r];
+        o.x *= sc;
+        o.y *= sc;
+        row[t + L * r] = o;
+      });
     }
   }
 }
 
-// Column pass: a tile of tc columns x G rows of one ring plane is staged in
-// LDS with coalesced row segments; each wave transforms whole columns in
-// registers; the tile is stored back with the source added and the field
-// sums of the tile reduced.
+// Column pass: a workgroup owns kTile adjacent columns of one ring plane;
+// each line slot of its waves loads one column straight from HBM (the
+// workgroup's loads of one row form one contiguous segment), transforms it
+// in registers, and stores it back with the source added and the field sums
+// of the tile reduced.
 template <class P, int G>
-__global__ __launch_bounds__(256) void fft_cols_reg_kernel(
-    typename C2<P>::T* u, int nslice, int tc, const double2* tw_d, const double2* mult_d,
+__global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
+    typename C2<P>::T* u, int nslice, const double2* tw_d, const double2* mult_d,
     const double* sigmoid, int mode, double scale, const typename C2<P>::T* src, const int4* rect,
     int first, int add_src, const double2* work, double* partials) {
   using T2 = typename C2<P>::T;
-  constexpr int R = RegShape<G>::R, L = RegShape<G>::L, LPW = RegShape<G>::LPW;
-  extern __shared__ unsigned char smem[];
-  T2* tw = reinterpret_cast<T2*>(smem);
-  T2* mu = tw + (G / 2 > 0 ? G / 2 : 1);
-  T2* lds = mu + G;
-  __shared__ double red[4][8];
-  const int pitch = tc + 1;
-  const int nt = blockDim.x;
-  const int ntile = (G + tc - 1) / tc;
+  using S = fftr::Shape<G>;
+  using C = RegCfg<P, G>;
+  constexpr int R = S::R, L = S::L;
+  constexpr int ntile = G / C::kTile;
+  __shared__ T2 tw[G];
+  __shared__ T2 mu[G];
+  __shared__ T2 buf[C::kWaves * S::LPW * S::BUF];
+  __shared__ double red[C::kWaves][8];
+  load_table(tw, tw_d, G);
+  if (mode == COL_PROP) load_table(mu, mult_d, G);
+  __syncthreads();
   const int p = blockIdx.x / ntile;
   const int tile = blockIdx.x - p * ntile;
-  const int c0 = tile * tc;
-  const int ncol = min(tc, G - c0);
-  load_tables(tw, mu, G, tw_d, mult_d, mode == COL_PROP);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L, t = lane % L;
+  const int cslot = wave * S::LPW + g;
+  const int c = tile * C::kTile + cslot;  // column of this line slot
+  const bool valid = cslot < C::kTile;
+  T2* lbuf = buf + cslot * S::BUF;
   T2* plane = u + static_cast<int64_t>(p) * G * G;
-  for (int e = threadIdx.x; e < tc * G; e += nt) {
-    const int r = e / tc, c = e - r * tc;
-    T2 z;
-    z.x = 0;
-    z.y = 0;
-    if (c < ncol) z = plane[static_cast<int64_t>(r) * G + c0 + c];
-    lds[r * pitch + c] = z;
+  T2 v[R];
+  fftr::static_for<0, R>([&](auto ic) {
+    constexpr int r = decltype(ic)::value;
+    v[r].x = 0;
+    v[r].y = 0;
+    if (valid) v[r] = plane[static_cast<int64_t>(t + L * r) * G + c];
+  });
+  fftr::forward<T2, G>(v, t, tw, lbuf);
+  if (mode == COL_PROP) {
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int q = decltype(ic)::value;
+      v[q] = fftr::cmul(v[q], mu[q * L + t]);
+    });
+  } else if (valid) {
+    // filter: this column holds x frequency pi(c), register q of lane t the
+    // y frequency pi(t + L q)
+    const int mx = fftr::spectrum_index(c, G);
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int q = decltype(ic)::value;
+      const int my = fftr::spectrum_index(t + L * q, G);
+      const P f = static_cast<P>(sigmoid[static_cast<int64_t>(my) * G + mx]);
+      v[q].x *= f;
+      v[q].y *= f;
+    });
   }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int t = lane % L;
-  const int wave = threadIdx.x >> 6, nwave = nt >> 6;
-  int lg = 0;
-  while ((1 << lg) < G) ++lg;
-  for (int cg = wave; cg * LPW < ncol; cg += nwave) {
-    const int c = cg * LPW + lane / L;
-    if (c >= ncol) continue;
-    T2 v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = lds[(t + L * r) * pitch + c];
-    reg_fft_fwd<T2, G>(v, t, tw);
-    if (mode == COL_PROP) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = cmul(v[r], mu[t + L * r]);
-    } else {
-      // filter: position ey holds my = bitrev(ey), column c holds mx = bitrev(c0+c)
-      const int mx = static_cast<int>(__brev(static_cast<unsigned>(c0 + c)) >> (32 - lg));
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int my = static_cast<int>(__brev(static_cast<unsigned>(t + L * r)) >> (32 - lg));
-        const P f = static_cast<P>(sigmoid[static_cast<int64_t>(my) * G + mx]);
-        v[r].x *= f;
-        v[r].y *= f;
-      }
-    }
-    reg_fft_inv<T2, G>(v, t, tw);
-#pragma unroll
-    for (int r = 0; r < R; ++r) lds[(t + L * r) * pitch + c] = v[r];
-  }
-  __syncthreads();
+  fftr::inverse<T2, G>(v, t, tw, lbuf);
 
   const P sc = static_cast<P>(scale);
   const int s = ((p - first) % nslice + nslice) % nslice;  // beam slice of this ring plane
@@ -530,55 +428,59 @@ __global__ __launch_bounds__(256) void fft_cols_reg_kernel(
   const T2* sp = (add_src == 1) ? src + static_cast<int64_t>(s) * G * G : nullptr;
   const double2* wp = (add_src == 2) ? work + static_cast<int64_t>(p) * G * G : nullptr;
   const double shift = -0.5 * static_cast<double>(G - 1);
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const int64_t icenter = (static_cast<int64_t>(G) * G - 1) / 2;
-  for (int e = threadIdx.x; e < ncol * G; e += nt) {
-    const int r = e / ncol, c = e - r * ncol;
-    const int ix = c0 + c;
-    T2 v = lds[r * pitch + c];
-    v.x *= sc;
-    v.y *= sc;
-    if (sp && ix >= rc.x && ix < rc.z && r >= rc.y && r < rc.w) {
-      const T2 sv = sp[static_cast<int64_t>(r) * G + ix];
-      v.x += P(2) * sv.x;
-      v.y += P(2) * sv.y;
-    } else if (wp) {
-      const double2 wv = wp[static_cast<int64_t>(r) * G + ix];
-      v.x += static_cast<P>(wv.x);
-      v.y += static_cast<P>(wv.y);
-    }
-    const int64_t gi = static_cast<int64_t>(r) * G + ix;
-    plane[gi] = v;
-    if (partials) {
-      const double re = v.x, im = v.y;
-      const double wei = re * re + im * im;
-      const double dx = ix + shift, dy = r + shift;
-      acc[0] += wei;
-      acc[1] += dx * wei;
-      acc[2] += dx * dx * wei;
-      acc[3] += dy * wei;
-      acc[4] += dy * dy * wei;
-      acc[5] += re;
-      acc[6] += im;
-      if (gi == icenter) {
-        partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
-        partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+  const bool src_col = sp && c >= rc.x && c < rc.z;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (valid) {
+    const double dx = c + shift;
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      const int row = t + L * r;
+      const int64_t gi = static_cast<int64_t>(row) * G + c;
+      T2 o = v[r];
+      o.x *= sc;
+      o.y *= sc;
+      if (src_col && row >= rc.y && row < rc.w) {
+        const T2 sv = sp[gi];
+        o.x += P(2) * sv.x;
+        o.y += P(2) * sv.y;
+      } else if (wp) {
+        const double2 wv = wp[gi];
+        o.x += static_cast<P>(wv.x);
+        o.y += static_cast<P>(wv.y);
       }
-    }
+      plane[gi] = o;
+      if (partials) {
+        const double re = o.x, im = o.y;
+        const double wei = re * re + im * im;
+        const double dy = row + shift;
+        acc[0] += wei;
+        acc[1] += dx * wei;
+        acc[2] += dx * dx * wei;
+        acc[3] += dy * wei;
+        acc[4] += dy * dy * wei;
+        acc[5] += re;
+        acc[6] += im;
+        if (gi == icenter) {
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+        }
+      }
+    });
   }
   if (partials) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      double v = acc[k];
+      double a = acc[k];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0) red[wave][k] = v;
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      if (lane == 0) red[wave][k] = a;
     }
     __syncthreads();
     if (threadIdx.x < 7) {
-      double v = 0.0;
-      for (int w = 0; w < nwave; ++w) v += red[w][threadIdx.x];
-      partials[(static_cast<int64_t>(p) * ntile + tile) * 8 + threadIdx.x] = v;
+      double a = 0.0;
+      for (int w = 0; w < C::kWaves; ++w) a += red[w][threadIdx.x];
+      partials[(static_cast<int64_t>(p) * ntile + tile) * 8 + threadIdx.x] = a;
     }
   }
 }
@@ -639,35 +541,26 @@ DftCfg dft_cfg(int G, size_t esz) {
   return c;
 }
 
-// Column tile width of the register kernels: 16 (float2) or 8 (double2)
-// columns, fewer when the tile would not fit 60 KiB of LDS.
-int reg_tc(int G, size_t esz) {
-  int tc = esz == 8 ? 16 : 8;
-  while (tc > 1 && (static_cast<size_t>(G) * (tc + 1) + G + G / 2) * esz > 60 * 1024) tc /= 2;
-  return std::min(tc, G);
-}
-
-int col_tile(int G, size_t esz) { return is_pow2(G) ? reg_tc(G, esz) : dft_cfg(G, esz).tc; }
-
 template <class P, int G>
 void rows_reg(Field* f, typename C2<P>::T* data, int mode, double scale) {
-  constexpr int LPW = RegShape<G>::LPW;
+  using C = RegCfg<P, G>;
+  constexpr int LPW = fftr::Shape<G>::LPW;
   const int64_t nrows = static_cast<int64_t>(f->nslice) * G;
   const int64_t ngroups = (nrows + LPW - 1) / LPW;
-  const int64_t nblk = std::min<int64_t>((ngroups + 3) / 4, 8192);
-  hipLaunchKernelGGL((fft_rows_reg_kernel<P, G>), dim3(static_cast<unsigned>(nblk)), dim3(256), 0,
-                     f->ctx->stream, data, nrows, f->fft_tw.get(), f->fft_prop.get(), mode, scale);
+  const int64_t nblk = std::min<int64_t>((ngroups + C::kWaves - 1) / C::kWaves, 8192);
+  hipLaunchKernelGGL((fft_rows_reg_kernel<P, G>), dim3(static_cast<unsigned>(nblk)),
+                     dim3(C::kThreads), 0, f->ctx->stream, data, nrows, f->fft_tw.get(),
+                     f->fft_prop.get(), mode, scale);
 }
 
 template <class P, int G>
 void cols_reg(Field* f, typename C2<P>::T* data, int mode, double scale, int add_src,
               double* partials) {
   using T2 = typename C2<P>::T;
-  const int tc = reg_tc(G, sizeof(T2));
-  const int ntile = (G + tc - 1) / tc;
-  const size_t lds = sizeof(T2) * (std::max(G / 2, 1) + G + static_cast<size_t>(G) * (tc + 1));
-  hipLaunchKernelGGL((fft_cols_reg_kernel<P, G>), dim3(f->nslice * ntile), dim3(256), lds,
-                     f->ctx->stream, data, f->nslice, tc, f->fft_tw.get(), f->fft_prop.get(),
+  using C = RegCfg<P, G>;
+  constexpr int ntile = G / C::kTile;
+  hipLaunchKernelGGL((fft_cols_reg_kernel<P, G>), dim3(f->nslice * ntile), dim3(C::kThreads), 0,
+                     f->ctx->stream, data, f->nslice, f->fft_tw.get(), f->fft_prop.get(),
                      f->fft_sigmoid.get(), mode, scale, static_cast<const T2*>(f->src()),
                      f->src_rect.get(), f->first, add_src, f->fft_work.get(), partials);
 }
@@ -688,6 +581,17 @@ void with_pow2(int G, Fn&& fn) {
     case 1024: fn(std::integral_constant<int, 1024>{}); break;
     default: throw Error(G4H_ERR_UNSUPPORTED, "FFT size is not a power of two in [2, 1024]");
   }
+}
+
+// Columns per workgroup tile of the column pass.
+int col_tile(int G, size_t esz) {
+  if (!is_pow2(G)) return dft_cfg(G, esz).tc;
+  int tc = 0;
+  with_pow2(G, [&](auto g) {
+    constexpr int gv = decltype(g)::value;
+    tc = esz == sizeof(double2) ? RegCfg<double, gv>::kTile : RegCfg<float, gv>::kTile;
+  });
+  return tc;
 }
 
 template <class P>
@@ -775,10 +679,17 @@ void fft_step(Field* f, bool source_on, bool stats) {
 void fft_prepare(Field* f, double delz) {
   const int G = f->ngrid;
   if (f->fft_tw.size() != static_cast<size_t>(G)) {
+    // Power-of-two sizes: the four-step middle twiddles W_G^(t*k2) at
+    // [k2*L + t]; other sizes: W_G^k at [k] for the direct DFT.
     std::vector<double2> tw(G);
     const double pi2 = 4.0 * std::asin(1.0);
     for (int k = 0; k < G; ++k) {
-      const double a = -pi2 * static_cast<double>(k) / static_cast<double>(G);
+      int64_t e = k;
+      if (is_pow2(G)) {
+        const int L = fftr::lanes_per_line(G);
+        e = static_cast<int64_t>(k % L) * (k / L);
+      }
+      const double a = -pi2 * static_cast<double>(e % G) / static_cast<double>(G);
       tw[k] = make_double2(std::cos(a), std::sin(a));
     }
     f->fft_tw.alloc(f->ctx, G);
@@ -809,18 +720,13 @@ void fft_prepare(Field* f, double delz) {
     // k2 at frequency index m: dx(m) = ((m - (G+1)/2) mod G) - (G-1)/2 and
     // k2 = -(dx^2 + dy^2) dk^2/(2 xks) (FieldSolverFFTCUDA.cpp:297-309);
     // the x (or y) factor is exp(-i dx^2 dk^2 delz / (2 xks)).
-    // Power-of-two sizes use the register transforms, whose spectrum is in
-    // bit-reversed order: entry e then holds the factor of frequency bitrev(e).
+    // Power-of-two sizes use the four-step register transforms, whose
+    // spectrum sits at permuted positions: entry e holds the factor of
+    // frequency spectrum_index(e).
     const double dk = 4. * std::asin(1.) / (static_cast<double>(G) * f->dgrid);
-    int lg = 0;
-    while ((1 << lg) < G) ++lg;
     std::vector<double2> pr(G);
     for (int e = 0; e < G; ++e) {
-      int m = e;
-      if (is_pow2(G)) {
-        m = 0;
-        for (int b = 0; b < lg; ++b) m |= ((e >> b) & 1) << (lg - 1 - b);
-      }
+      const int m = is_pow2(G) ? fftr::spectrum_index(e, G) : e;
       const int ix = ((m - (G + 1) / 2) % G + G) % G;
       const double dx = static_cast<double>(ix) - 0.5 * static_cast<double>(G - 1);
       const double ph = -dx * dx * dk * dk / 2. / f->xks * delz;

@@ -180,7 +180,7 @@ struct Field {
   DevBuf<double2> srcd;
   DevBuf<float2> srcf;
   DevBuf<int4> src_rect;  // [nslice] half-open (x0,y0,x1,y1) of possibly nonzero cells
-  DevBuf<int4> dep_bbox;  // [nslice] (minx,miny,maxx,maxy) of deposited lower-left cells
+  DevBuf<int4> dep_bbox;  // [nslice] (minx,miny,maxx,maxy) of the cells deposited last step
   // solver
   g4h_solver_kind kind = G4H_SOLVER_ADI;
   bool filter = false;

@@ -88,8 +88,8 @@ class Beam {
   Beam(Context& ctx, g4h_precision prec, std::vector<int32_t> slice_offsets,
        std::vector<double> current_in, double gamma_ref, double reflength_in,
        double slicelength_in)
-      : ctx_(ctx), offsets(std::move(slice_offsets)), current(std::move(current_in)),
-        reflength(reflength_in), slicelength(slicelength_in), gamma_ref(gamma_ref) {
+      : offsets(std::move(slice_offsets)), current(std::move(current_in)),
+        reflength(reflength_in), slicelength(slicelength_in), gamma_ref(gamma_ref), ctx_(ctx) {
     check(g4h_beam_create(ctx.get(), prec, static_cast<int>(offsets.size()) - 1, offsets.data(),
                           current.data(), gamma_ref, &soa_));
   }
@@ -139,10 +139,9 @@ class Field {
  public:
   Field(Context& ctx, g4h_precision prec, int nslice, int ngrid, double gridmax,
         double xlambda0, double slicelength, int harm = 1)
-      : ctx_(ctx), ngrid(ngrid), harm(harm), gridmax(gridmax),
-        dgrid(2 * gridmax / (ngrid - 1)), xlambda(xlambda0 / harm),
-        xks(2.0 * 3.14159265358979323846 / (xlambda0 / harm)), slicelength(slicelength),
-        nslice_(nslice) {
+      : ngrid(ngrid), harm(harm), gridmax(gridmax), dgrid(2 * gridmax / (ngrid - 1)),
+        xlambda(xlambda0 / harm), xks(2.0 * 3.14159265358979323846 / (xlambda0 / harm)),
+        slicelength(slicelength), ctx_(ctx), nslice_(nslice) {
     check(g4h_field_create(ctx.get(), prec, nslice, ngrid, gridmax, xlambda0, slicelength, harm,
                            &soa_));
   }

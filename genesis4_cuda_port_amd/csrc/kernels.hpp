@@ -30,7 +30,7 @@ struct TransParams {
 
 struct GatherField {
   const void* data;  // double2* or float2*, ring storage
-  double gridmax, dgrid, rtmp, rharm;
+  double gridmax, dgrid, inv_dgrid, rtmp, rharm;
   int first, ngrid, nslice, pad_;
 };
 
@@ -51,7 +51,7 @@ struct DepParams {
   int4* bbox;                   // [nslice]
   const double* cur_over_npar;  // [nslice]
   double scl_factor;            // vacimp*xks*fc*delz/(4*eev*dgrid^2)
-  double gridmax, dgrid;
+  double gridmax, dgrid, inv_dgrid;
   double ax, ay, kx, ky, gradx, grady;
   int ngrid, harm;
   int window_cap;  // cells of the LDS window

@@ -35,7 +35,7 @@
 namespace g4 {
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxHarm = 8;
 constexpr int kNMinMax = 10;
@@ -170,17 +170,34 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
 // ---- particle -> cell (BeamSolverCUDA.cpp:99-116, FieldSolverADICUDA.cpp:
 // 126-143): lower-left cell (ix, iy) and the weight wx, wy of that cell's
 // column/row.  The gather, the deposition and g4h_beam_cell_index share it.
+// FP64 evaluates the reference's (x + gm)/dg.  FP32 uses gm/dg = (ng-1)/2
+// exactly and the correctly rounded 1/dg: the cell coordinate is one FMA and
+// the in-cell fraction a second FMA against the exact half-integer
+// (ng-1)/2 - ix, so the weights keep FP32 precision even where the
+// coordinate itself is ~ng/2 (a direct FP32 (x+gm)/dg would leave only
+// ~2^-24 * ng/2 of a cell, 3e-5 at ng = 1024).
 template <class P>
-__device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, int ng, int& ix, int& iy, P& wx,
-                                        P& wy) {
+__device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, P idg, int ng, int& ix, int& iy,
+                                        P& wx, P& wy) {
   if ((x <= -gm) || (x >= gm) || (y <= -gm) || (y >= gm)) return false;
-  const P wxr = (x + gm) / dg;
-  const P wyr = (y + gm) / dg;
-  ix = static_cast<int>(floor(wxr));
-  iy = static_cast<int>(floor(wyr));
-  if (ix < 0 || ix >= ng - 1 || iy < 0 || iy >= ng - 1) return false;
-  wx = P(1) + static_cast<P>(ix) - wxr;
-  wy = P(1) + static_cast<P>(iy) - wyr;
+  if constexpr (sizeof(P) == 8) {
+    (void)idg;
+    const P wxr = (x + gm) / dg;
+    const P wyr = (y + gm) / dg;
+    ix = static_cast<int>(floor(wxr));
+    iy = static_cast<int>(floor(wyr));
+    if (ix < 0 || ix >= ng - 1 || iy < 0 || iy >= ng - 1) return false;
+    wx = P(1) + static_cast<P>(ix) - wxr;
+    wy = P(1) + static_cast<P>(iy) - wyr;
+  } else {
+    (void)dg;
+    const float half = 0.5f * static_cast<float>(ng - 1);
+    ix = static_cast<int>(floorf(fmaf(x, idg, half)));
+    iy = static_cast<int>(floorf(fmaf(y, idg, half)));
+    if (ix < 0 || ix >= ng - 1 || iy < 0 || iy >= ng - 1) return false;
+    wx = 1.0f - fmaf(x, idg, half - static_cast<float>(ix));
+    wy = 1.0f - fmaf(y, idg, half - static_cast<float>(iy));
+  }
   return true;
 }
 
@@ -194,7 +211,7 @@ __device__ __forceinline__ void gather(const GatherField& f, int slice, P x, P y
   const int ng = f.ngrid;
   int ix, iy;
   P wx, wy;
-  if (!cell_of<P>(x, y, P(f.gridmax), P(f.dgrid), ng, ix, iy, wx, wy)) return;
+  if (!cell_of<P>(x, y, P(f.gridmax), P(f.dgrid), P(f.inv_dgrid), ng, ix, iy, wx, wy)) return;
   const int islice = (slice + f.first) % f.nslice;
   const T2* u = static_cast<const T2*>(f.data) + static_cast<int64_t>(islice) * ng * ng +
                 ix + static_cast<int64_t>(iy) * ng;
@@ -390,12 +407,16 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Per-particle work of the pass, shared by the prefetching loop below.
-template <class P, int NF, bool DEP, int NH>
+// Per-thread partial sums of the beam diagnostics.  The energy and bunching
+// sums are FP64 in both modes.  The ten transverse moments are FP64 in FP64
+// mode and, in MIXED mode, FP32 over the thread's own particles (a slice's
+// particles / kThreads of them); every reduction across threads, waves and
+// slices is FP64.
+template <class P, int NH>
 struct PassState {
-  double acc[NH > 0 ? 13 + 2 * NH : 1];
-  P mm[NH > 0 ? 10 : 1];
-  int bminx = 0x7fffffff, bminy = 0x7fffffff, bmaxx = -1, bmaxy = -1;
+  double e[NH > 0 ? 2 + 2 * NH : 1];  // dgamma, dgamma^2, then (cos, sin) per harmonic
+  P m[NH > 0 ? 10 : 1];               // x, x^2, y, y^2, px, px^2, py, py^2, x px, y py
+  P mm[NH > 0 ? 10 : 1];              // xmin, xmax, pxmin, pxmax, ymin, ymax, pymin, pymax, gmin, gmax
 };
 
 template <class P, int NF, bool DEP, int NH>
@@ -403,7 +424,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
                                         int64_t i, BeamPtr<P> b, P x, P y, P px, P py, P g,
                                         double th, P scl, typename V2<P>::T* src, double2* win,
                                         int wx0, int wy0, int ww, int wh, int* s_fb,
-                                        PassState<P, NF, DEP, NH>& ps) {
+                                        PassState<P, NH>& ps) {
   const int st = prm.stages;
   const double g0 = prm.gamma0;
   const int ng = prm.dep.ngrid;
@@ -453,7 +474,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
       const DepParams& d = prm.dep;
       int ix, iy;
       P wx, wy;
-      if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), ng, ix, iy, wx, wy)) {
+      if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), P(d.inv_dgrid), ng, ix, iy, wx, wy)) {
         const P dx = x - P(d.ax), dy = y - P(d.ay);
         const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
                        P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
@@ -474,10 +495,6 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
         const P cre = sn * part;
         const P cim = cs * part;
         const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
-        ps.bminx = min(ps.bminx, ix);
-        ps.bminy = min(ps.bminy, iy);
-        ps.bmaxx = max(ps.bmaxx, ix);
-        ps.bmaxy = max(ps.bmaxy, iy);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const int cx = ix + (k & 1), cy = iy + (k >> 1);
@@ -501,22 +518,20 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   }
 
   if constexpr (NH > 0) {
-    const double xv = x, yv = y, pxv = px, pyv = py;
     const double dg = gshift(g, g0);
-    double* acc = ps.acc;
-    acc[0] += dg;
-    acc[1] += dg * dg;
-    acc[2] += xv;
-    acc[3] += xv * xv;
-    acc[4] += yv;
-    acc[5] += yv * yv;
-    acc[6] += pxv;
-    acc[7] += pxv * pxv;
-    acc[8] += pyv;
-    acc[9] += pyv * pyv;
-    acc[10] += xv * pxv;
-    acc[11] += yv * pyv;
-    acc[12] += 1.0;
+    ps.e[0] += dg;
+    ps.e[1] += dg * dg;
+    P* m = ps.m;
+    m[0] += x;
+    m[1] += x * x;
+    m[2] += y;
+    m[3] += y * y;
+    m[4] += px;
+    m[5] += px * px;
+    m[6] += py;
+    m[7] += py * py;
+    m[8] += x * px;
+    m[9] += y * py;
     // min/max in the storage type (exact); MIXED energy as gamma - gamma_ref
     P* mm = ps.mm;
     mm[0] = fmin(mm[0], x);
@@ -530,21 +545,21 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
     mm[8] = fmin(mm[8], g);
     mm[9] = fmax(mm[9], g);
     phase1();
-    acc[13] += cs1;
-    acc[14] += sn1;
+    ps.e[2] += cs1;
+    ps.e[3] += sn1;
 #pragma unroll
     for (int h = 1; h < NH; ++h) {
       if (h < prm.diag.nharm) {
         if constexpr (sizeof(P) == 8) {
           double sn, cs;
           sincos(static_cast<double>(h + 1) * th, &sn, &cs);
-          acc[13 + 2 * h] += cs;
-          acc[14 + 2 * h] += sn;
+          ps.e[2 + 2 * h] += cs;
+          ps.e[3 + 2 * h] += sn;
         } else {
           float sn, cs;
           fsincos(static_cast<double>(h + 1) * th, sn, cs);
-          acc[13 + 2 * h] += cs;
-          acc[14 + 2 * h] += sn;
+          ps.e[2 + 2 * h] += cs;
+          ps.e[3 + 2 * h] += sn;
         }
       }
     }
@@ -555,11 +570,11 @@ template <class P, int NF, bool DEP, int NH>
 __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
                                                             ParticleParams prm) {
   using T2 = typename V2<P>::T;
-  constexpr int NS = NH > 0 ? 13 + 2 * NH : 1;
+  constexpr int NSUM = NH > 0 ? 12 + 2 * NH : 1;  // sums other than the count
   extern __shared__ double2 win[];
-  __shared__ int s_bb[4];   // particle bbox min x, min y, max x, max y (lower-left cells)
-  __shared__ int s_fb[4];   // fallback (outside-window) touched-cell bbox
-  __shared__ double s_red[kWaves][NS + kNMinMax];
+  __shared__ int s_bb[4];  // touched-cell bbox (min x, min y, max x, max y), inclusive
+  __shared__ int s_fb[4];  // same, for cells reached through the global fallback
+  __shared__ double s_red[kWaves][NSUM + kNMinMax];
 
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
@@ -567,6 +582,8 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   const double g0 = prm.gamma0;
 
   // ---- deposition set-up -------------------------------------------------
+  // The LDS window covers the cells the slice touched last step (dep.bbox)
+  // plus a 2-cell margin, clipped to the grid and to window_cap cells.
   const int ng = prm.dep.ngrid;
   int wx0 = 0, wy0 = 0, ww = 0, wh = 0;
   P scl = P(0);
@@ -574,10 +591,39 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   if constexpr (DEP) {
     src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
     scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
-    const int4 bb = prm.dep.bbox[s];
+    int4 bb = prm.dep.bbox[s];
+    if (bb.x > bb.z && scl != P(0)) {
+      // No footprint from a previous step (first deposition of a run): take
+      // it from the current positions, which move by far less than the
+      // window margin within one step.
+      if (tid == 0) {
+        s_bb[0] = s_bb[1] = 0x7fffffff;
+        s_bb[2] = s_bb[3] = -1;
+      }
+      __syncthreads();
+      int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+      for (int64_t k = i0 + tid; k < i1; k += kThreads) {
+        int ix, iy;
+        P wx, wy;
+        if (cell_of<P>(b.x[k], b.y[k], P(prm.dep.gridmax), P(prm.dep.dgrid),
+                       P(prm.dep.inv_dgrid), ng, ix, iy, wx, wy)) {
+          mnx = min(mnx, ix);
+          mny = min(mny, iy);
+          mxx = max(mxx, ix + 1);
+          mxy = max(mxy, iy + 1);
+        }
+      }
+      atomicMin(&s_bb[0], mnx);
+      atomicMin(&s_bb[1], mny);
+      atomicMax(&s_bb[2], mxx);
+      atomicMax(&s_bb[3], mxy);
+      __syncthreads();
+      bb = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
+      __syncthreads();
+    }
     if (bb.x <= bb.z && scl != P(0)) {
       int x0 = max(0, bb.x - 2), y0 = max(0, bb.y - 2);
-      int x1 = min(ng, bb.z + 4), y1 = min(ng, bb.w + 4);
+      int x1 = min(ng, bb.z + 3), y1 = min(ng, bb.w + 3);
       int w = x1 - x0, h = y1 - y0;
       const int cap = prm.dep.window_cap;
       if (w * h > cap) {
@@ -617,10 +663,12 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     __syncthreads();
   }
 
-  PassState<P, NF, DEP, NH> ps;
+  PassState<P, NH> ps;
   if constexpr (NH > 0) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) ps.acc[k] = 0.0;
+    for (int k = 0; k < 2 + 2 * NH; ++k) ps.e[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) ps.m[k] = P(0);
     const P big = P(1.0e5);
     ps.mm[0] = big;  ps.mm[1] = -big;  // x
     ps.mm[2] = big;  ps.mm[3] = -big;  // px
@@ -675,11 +723,10 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   }
 
   if constexpr (DEP) {
-    atomicMin(&s_bb[0], ps.bminx);
-    atomicMin(&s_bb[1], ps.bminy);
-    atomicMax(&s_bb[2], ps.bmaxx);
-    atomicMax(&s_bb[3], ps.bmaxy);
     __syncthreads();
+    // write the window with plain stores; its non-zero cells give the
+    // footprint for the next step's window
+    int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
     for (int k = tid; k < ww * wh; k += kThreads) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
       const double2 v = win[k];
@@ -687,7 +734,18 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       o.x = static_cast<P>(v.x);
       o.y = static_cast<P>(v.y);
       src[cx + static_cast<int64_t>(cy) * ng] = o;
+      if (v.x != 0.0 || v.y != 0.0) {
+        mnx = min(mnx, cx);
+        mny = min(mny, cy);
+        mxx = max(mxx, cx);
+        mxy = max(mxy, cy);
+      }
     }
+    atomicMin(&s_bb[0], mnx);
+    atomicMin(&s_bb[1], mny);
+    atomicMax(&s_bb[2], mxx);
+    atomicMax(&s_bb[3], mxy);
+    __syncthreads();
     if (tid == 0) {
       int4 r;
       if (ww > 0 && wh > 0) {
@@ -695,26 +753,32 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       } else {
         r = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
       }
+      int4 t = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
       if (s_fb[0] <= s_fb[2]) {
         r.x = min(r.x, s_fb[0]);
         r.y = min(r.y, s_fb[1]);
         r.z = max(r.z, s_fb[2] + 1);
         r.w = max(r.w, s_fb[3] + 1);
+        t.x = min(t.x, s_fb[0]);
+        t.y = min(t.y, s_fb[1]);
+        t.z = max(t.z, s_fb[2]);
+        t.w = max(t.w, s_fb[3]);
       }
       if (r.x > r.z) r = make_int4(0, 0, 0, 0);
       prm.dep.rect[s] = r;
-      prm.dep.bbox[s] = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
+      prm.dep.bbox[s] = t;
     }
   }
 
   if constexpr (NH > 0) {
     const int nharm = prm.diag.nharm;
     const int lane = tid & 63, wave = tid >> 6;
-    const int nsum = 13 + 2 * nharm;
+    const int nsum = 12 + 2 * nharm;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
+    for (int k = 0; k < NSUM; ++k) {
       if (k < nsum) {
-        const double v = wave_sum(ps.acc[k]);
+        const double a = k < 2 ? ps.e[k] : (k < 12 ? static_cast<double>(ps.m[k - 2]) : ps.e[k - 10]);
+        const double v = wave_sum(a);
         if (lane == 0) s_red[wave][k] = v;
       }
     }
@@ -722,21 +786,24 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     for (int k = 0; k < kNMinMax; ++k) {
       const double m = static_cast<double>(ps.mm[k]);
       const double v = (k & 1) ? wave_max(m) : wave_min(m);
-      if (lane == 0) s_red[wave][NS + k] = v;
+      if (lane == 0) s_red[wave][NSUM + k] = v;
     }
     __syncthreads();
     double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
-    // raw layout: 13 sums (G4H_B_G1..G4H_B_COUNT), 10 min/max, then bunching
+    // raw layout: 12 sums (G4H_B_G1..G4H_B_YPY), the count, 10 min/max, then
+    // (cos, sin) sums per harmonic
     if (tid < nsum) {
       double v = 0.0;
       for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
-      const int dst = tid < 13 ? tid : G4H_BEAM_STATS + (tid - 13);
+      const int dst = tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12);
       out[dst] = v;
-    } else if (tid >= 64 && tid < 64 + kNMinMax) {
-      const int k = tid - 64;
-      double v = s_red[0][NS + k];
+    } else if (tid == 64) {
+      out[G4H_B_COUNT] = static_cast<double>(i1 - i0);
+    } else if (tid >= 128 && tid < 128 + kNMinMax) {
+      const int k = tid - 128;
+      double v = s_red[0][NSUM + k];
       for (int w = 1; w < kWaves; ++w)
-        v = (k & 1) ? fmax(v, s_red[w][NS + k]) : fmin(v, s_red[w][NS + k]);
+        v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
       if (sizeof(P) == 4 && k >= 8) v += g0;  // MIXED energy extremes were shifted
       // order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
       out[G4H_B_XMIN + k] = v;
@@ -751,7 +818,9 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
   if (i >= n) return;
   int ix, iy;
   P wx, wy;
-  cell[i] = cell_of<P>(b.x[i], b.y[i], P(gm), P(dg), ng, ix, iy, wx, wy) ? ix + iy * ng : -1;
+  cell[i] = cell_of<P>(b.x[i], b.y[i], P(gm), P(dg), P(1.0 / dg), ng, ix, iy, wx, wy)
+                ? ix + iy * ng
+                : -1;
 }
 
 template <class P, int NF, bool DEP, int NH>

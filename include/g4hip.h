@@ -273,6 +273,40 @@ g4h_status g4h_lattice_fodo(double lambdau, const double* aw_mod, int nmod, int 
                             int quad_spacing_periods, int quad_len_periods, double k1,
                             int output_step, g4h_step* steps, int capacity, int* nsteps);
 
+/* ---- input deck and lattice file --------------------------------------
+ * The reference's driver reads a namelist deck (&setup, &lattice, &time,
+ * &field, &beam, &track; src/Main/Track.cpp:62-134, example test/fel.in) and
+ * a lattice file (test/fel.lat) through upstream GENESIS, which is not part
+ * of the reference.  These entry points serve that front-end.              */
+
+/* Lattice file text -> per-step table of beamline `beamline` (UNDULATOR,
+ * DRIFT, QUADRUPOLE, CORRECTOR, CHICANE, PHASESHIFTER, MARKER, LINE with
+ * n*label repeats): undulators cut into steps of about delz, other elements
+ * one step each, slippage in reference wavelengths, outstep every
+ * output_step steps, truncated at zstop.  Writes min(capacity, needed) steps
+ * and returns the count needed in *nsteps. */
+g4h_status g4h_lattice_parse(const char* lattice_text, const char* beamline, double delz,
+                             double lambda0, double gamma0, int output_step, double zstop,
+                             g4h_step* steps, int capacity, int* nsteps);
+
+/* Periodic (matched) Twiss parameters of the section [0, zmatch] of a step
+ * table (&lattice zmatch): twiss = {betax, alphax, betay, alphay}. */
+g4h_status g4h_lattice_match(const g4h_step* steps, int nsteps, double gamma0, double zmatch,
+                             double* twiss);
+
+/* Namelist deck text -> JSON [{"name": ..., "keys": [[key, value], ...]}, ...]
+ * (lower-case names and keys, raw values, file order).  Writes at most
+ * capacity bytes (NUL-terminated) and the size needed in *needed. */
+g4h_status g4h_deck_parse(const char* deck_text, char* json, int capacity, int* needed);
+
+/* Runs a deck end to end on GPU `device` (the driver of src/Main/Track.cpp
+ * with &track use_gpu, alias use_cuda; use_gpu = false returns
+ * G4H_ERR_UNSUPPORTED since the CPU path is upstream GENESIS) and writes the
+ * records as .npy arrays named like the reference's output groups
+ * (out_dir/Beam/energy.npy, out_dir/Field/power.npy, ..., shape
+ * [records, slices]) plus out_dir/meta.json.  *nrecords: records written. */
+g4h_status g4h_deck_run(const char* deck_path, const char* out_dir, int device, int* nrecords);
+
 #ifdef __cplusplus
 }
 #endif

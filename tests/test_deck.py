@@ -1,0 +1,176 @@
+"""CPU tests of the input deck / lattice file front-end (no GPU needed).
+
+The reference's own example input (test/fel.in, test/fel.lat, copied under
+tests/golden/reference_example/) is the fixture; the step table is checked
+against closed forms (resonance condition, chicane path length) and the
+matched optics against an independent numpy propagation of the Twiss
+parameters through the same section.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from genesis4_cuda_port_amd import api
+from genesis4_cuda_port_amd._abi import G4Error
+
+EX = Path(__file__).resolve().parent / "golden" / "reference_example"
+LAMBDA0, GAMMA0, DELZ = 0.15e-9, 27984.34, 0.24
+
+
+def test_deck_parse_reference_example():
+    nl = api.deck_parse((EX / "fel.in").read_text())
+    assert [n for n, _ in nl] == ["setup", "lattice", "time", "field", "beam", "track"]
+    setup = dict(nl[0][1])
+    assert setup["lattice"] == "fel.lat" and setup["beamline"] == "FEL"
+    assert float(setup["lambda0"]) == 0.15e-9 and int(setup["npart"]) == 4800
+    assert dict(nl[1][1]) == {"zmatch": "5.72"}
+    assert dict(nl[5][1]) == {"use_cuda": "1", "low_memory_adisolver": "1"}
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("&setup\nx=1\n", "not closed"),
+    ("x = 1\n", "outside a namelist"),
+    ("&setup\nx\n&end\n", "expected key = value"),
+    ("&setup\n&beam\n&end\n", "not closed before"),
+    ("&end\n", "without an open namelist"),
+])
+def test_deck_parse_errors_name_the_line(text, msg):
+    with pytest.raises(G4Error) as e:
+        api.deck_parse(text)
+    assert msg in str(e.value) and "line" in str(e.value)
+
+
+def test_deck_parse_comments_and_separators():
+    nl = api.deck_parse("# c\n&Track ! x\n a = 1, b='x#y' ; C=.true.\n&END\n")
+    assert nl == [("track", [("a", "1"), ("b", "'x#y'"), ("c", ".true.")])]
+
+
+def _steps(**kw):
+    lat = (EX / "fel.lat").read_text()
+    return api.lattice_parse(lat, kw.pop("beamline", "FEL"), DELZ, LAMBDA0, GAMMA0, **kw)
+
+
+def test_lattice_reference_example_step_table():
+    st = _steps()
+    aw = np.array([s.aw for s in st])
+    dz = np.array([s.delz for s in st])
+    z = np.array([s.z for s in st])
+    # FEL = 10 x FODO_D + marker; FODO_D = 2 x (UND 72 x 3 cm, 0.3 m drift,
+    # 0.1 m quad, 0.1 m drift, 0.1 m chicane, 0.1 m drift)
+    assert len(st) == 10 * 2 * (9 + 5)
+    assert np.count_nonzero(aw) == 180 and np.allclose(dz[aw > 0], 0.24)
+    assert z[-1] == pytest.approx(10 * 5.72, rel=1e-12)
+    assert np.allclose(np.diff(np.concatenate([[0], z])), dz)
+    qf = np.array([s.qf for s in st])
+    assert sorted(set(qf[qf != 0])) == [-1.0, 1.01]
+    assert st[len(st) - 1].marker == 3  # dumpbeam + dumpfield on the last step
+    assert all(s.outstep == 1 for s in st)
+    ku = 2 * np.pi / 0.03
+    u = [s for s in st if s.aw > 0]
+    assert all(s.ku == pytest.approx(ku) and s.ky == pytest.approx(ku * ku) and s.kx == 0 for s in u)
+    # resonance: an undulator step slips (periods) x (resonant wavelength / lambda0)
+    lr = 0.03 * (1 + 2.613 ** 2) / (2 * GAMMA0 ** 2)
+    assert u[0].slippage == pytest.approx(8 * lr / LAMBDA0, rel=1e-12)
+    # drifts slip l / (2 gamma0^2 lambda0)
+    d = [s for s in st if s.aw == 0 and s.qf == 0 and s.chic_angle == 0]
+    assert d[0].slippage == pytest.approx(0.3 / (2 * GAMMA0 ** 2 * LAMBDA0), rel=1e-12)
+
+
+def test_lattice_chicane_angle_solves_the_delay():
+    st = [s for s in _steps() if s.chic_angle != 0]
+    assert len(st) == 20
+    s = st[0]
+    th, lb, ld = s.chic_angle, s.chic_lb, s.chic_ld
+    delay = 2 * lb * (th / np.sin(th) - 1) + 2 * ld * (1 / np.cos(th) - 1)
+    assert delay == pytest.approx(342e-10, rel=1e-9)
+    # small-angle: delay = lb theta^2 / 3 (+ ld theta^2)
+    assert th == pytest.approx(np.sqrt(3 * 342e-10 / 0.02), rel=1e-5)
+    assert s.slippage == pytest.approx(342e-10 / LAMBDA0 + 0.1 / (2 * GAMMA0 ** 2 * LAMBDA0))
+
+
+def test_lattice_output_step_and_zstop():
+    st = _steps(output_step=3, zstop=5.0)
+    assert st[len(st) - 1].z >= 5.0 and st[len(st) - 2].z < 5.0
+    out = [i for i, s in enumerate(st) if s.outstep]
+    assert out[:3] == [2, 5, 8] and out[-1] == len(st) - 1
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("A: LINE = {B};", "unknown label 'B'"),
+    ("A: LINE = {A};", "contains itself"),
+    ("A: WIGGLER = {l=1};", "unknown type"),
+    ("A: DRIFT = {l=1}; A: DRIFT = {l=2};", "defined twice"),
+    ("A: DRIFT = {l=x};", "expects a number"),
+])
+def test_lattice_errors(text, msg):
+    with pytest.raises(G4Error) as e:
+        api.lattice_parse(text, "A", DELZ, LAMBDA0, GAMMA0)
+    assert msg in str(e.value)
+
+
+def _twiss_transport(steps, gamma0, zmatch, b0, a0, plane):
+    """Propagate (beta, alpha) through the steps with numpy 2x2 matrices."""
+    g0 = (1 + a0 * a0) / b0
+    sig = np.array([[b0, -a0], [-a0, g0]])
+    for s in steps:
+        if s.z > zmatch + 1e-12:
+            break
+        aw2 = s.aw ** 2
+        gz = np.sqrt(gamma0 ** 2 - 1 - aw2)
+        bpar = np.sqrt(1 - (1 + aw2) / gamma0 ** 2)
+        k = (s.qf * gamma0 + s.kx * aw2 / (gamma0 * bpar)) if plane == "x" else \
+            (-s.qf * gamma0 + s.ky * aw2 / (gamma0 * bpar))
+        k /= gz
+        L = s.delz
+        if k > 0:
+            w = np.sqrt(k)
+            m = np.array([[np.cos(w * L), np.sin(w * L) / w], [-w * np.sin(w * L), np.cos(w * L)]])
+        elif k < 0:
+            w = np.sqrt(-k)
+            m = np.array([[np.cosh(w * L), np.sinh(w * L) / w], [w * np.sinh(w * L), np.cosh(w * L)]])
+        else:
+            m = np.array([[1.0, L], [0.0, 1.0]])
+        sig = m @ sig @ m.T
+    return sig[0, 0], -sig[0, 1]
+
+
+def test_matched_optics_are_periodic():
+    st = _steps()
+    bx, ax, by, ay = api.lattice_match(st, GAMMA0, 5.72)
+    assert 1.0 < bx < 100.0 and 1.0 < by < 100.0
+    for b0, a0, pl in ((bx, ax, "x"), (by, ay, "y")):
+        b1, a1 = _twiss_transport(st, GAMMA0, 5.72, b0, a0, pl)
+        assert b1 == pytest.approx(b0, rel=1e-9) and a1 == pytest.approx(a0, abs=1e-9 * b0)
+    # a drift-only section has no periodic solution
+    d = api.lattice_parse("D: DRIFT = {l=1}; L: LINE = {3*D};", "L", DELZ, LAMBDA0, GAMMA0)
+    with pytest.raises(G4Error):
+        api.lattice_match(d, GAMMA0, 3.0)
+
+
+def test_deck_run_reports_unsupported_and_missing_device(tmp_path):
+    deck = tmp_path / "d.in"
+    deck.write_text("&setup\nrootname=x\nlattice=l.lat\nbeamline=L\nlambda0=1e-9\ngamma0=1e4\n"
+                    "delz=0.1\n&end\n&profile_gauss\nlabel=a\n&end\n")
+    n = C.c_int()
+    from genesis4_cuda_port_amd._abi import lib
+    assert lib().g4h_deck_run(str(deck).encode(), str(tmp_path).encode(), 0, C.byref(n)) == 5
+    assert b"profile_gauss" in lib().g4h_last_error()
+    deck.write_text("&setup\nrootname=x\nlattice=l.lat\nbeamline=L\nlambda0=1e-9\ngamma0=1e4\n"
+                    "delz=0.1\nbogus=3\n&end\n")
+    assert lib().g4h_deck_run(str(deck).encode(), str(tmp_path).encode(), 0, C.byref(n)) == 1
+    assert b"unknown key 'bogus'" in lib().g4h_last_error()
+
+
+def test_g4run_cli_usage_and_errors(tmp_path):
+    import subprocess
+    cli = Path(api.__file__).resolve().parent / "_lib" / "g4run"
+    res = subprocess.run([str(cli)], capture_output=True, text=True)
+    assert res.returncode == 2 and "usage" in res.stderr
+    deck = tmp_path / "d.in"
+    deck.write_text("&setup\nrootname=x\n&end\n&nonsense\n&end\n")
+    res = subprocess.run([str(cli), str(deck)], capture_output=True, text=True)
+    assert res.returncode == 1 and "g4run:" in res.stderr

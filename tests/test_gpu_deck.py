@@ -1,0 +1,95 @@
+"""GPU: the reference's example deck (test/fel.in + test/fel.lat, shortened
+time window and lattice) run end to end through g4h_deck_run, against the
+FP64 oracle fed with the same step table, beam and field.
+
+Bar: FP64 slice power and bunching within 1e-10 relative, as for the other
+FP64 runs."""
+from __future__ import annotations
+
+import json
+import shutil
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import _oracle
+from _oracle import OField
+from genesis4_cuda_port_amd import api
+
+pytestmark = pytest.mark.gpu
+
+EX = Path(__file__).resolve().parent / "golden" / "reference_example"
+LAMBDA0, GAMMA0, SAMPLE, ZSTOP = 0.15e-9, 27984.34, 8, 6.0
+NSLICE = 10
+
+
+def relerr(a, b):
+    den = np.linalg.norm(np.ravel(b))
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / (den if den > 0 else 1.0)
+
+
+def _deck(tmp_path: Path, solver_keys: str) -> Path:
+    text = (EX / "fel.in").read_text()
+    text = text.replace("slen=2.0e-6", f"slen={NSLICE * SAMPLE * LAMBDA0!r}")
+    text = text.replace("use_cuda=1", f"use_cuda=1\nzstop={ZSTOP}\nprecision=fp64\n{solver_keys}")
+    shutil.copy(EX / "fel.lat", tmp_path / "fel.lat")
+    deck = tmp_path / "fel.in"
+    deck.write_text(text)
+    return deck
+
+
+@pytest.mark.parametrize("fft", [False, True])
+def test_reference_deck_matches_oracle(tmp_path, fft):
+    print(f"case deck fft={fft}", flush=True)
+    deck = _deck(tmp_path, "fft_fieldsolver=1" if fft else "")
+    out = tmp_path / "out"
+    nrec = api.deck_run(deck, out)
+    meta = json.loads((out / "meta.json").read_text())
+    assert meta["nslice"] == NSLICE and meta["nrecords"] == nrec
+    # the same inputs, built through the front-end calls and run on the oracle
+    steps = api.lattice_parse((EX / "fel.lat").read_text(), "FEL", 0.24, LAMBDA0, GAMMA0, 1, ZSTOP)
+    assert nrec == 1 + len(steps)
+    beam = api.load_beam(nslice=NSLICE, npart=4800, nbins=4, gamma0=GAMMA0,
+                         espread_rel=2.24 / GAMMA0, emitn=1.2e-6, beta=meta["betax"],
+                         alpha=meta["alphax"], beta_y=meta["betay"], alpha_y=meta["alphay"],
+                         slicelength=SAMPLE * LAMBDA0, current=3400.0, shotnoise=True,
+                         seed=1234567)
+    bx, ax, by, ay = api.lattice_match(api.lattice_parse((EX / "fel.lat").read_text(), "FEL",
+                                                         0.24, LAMBDA0, GAMMA0), GAMMA0, 5.72)
+    assert (bx, ax, by, ay) == pytest.approx((meta["betax"], meta["alphax"], meta["betay"],
+                                              meta["alphay"]), rel=1e-12, abs=1e-12)
+    of = OField(NSLICE, 127, 120e-6, LAMBDA0, SAMPLE * LAMBDA0)
+    _, braw, fraw = _oracle.run(beam, of, steps, gamma0=GAMMA0, lambda0=LAMBDA0,
+                                solver=api.FFT if fft else api.ADI, sample=SAMPLE, timerun=True,
+                                nharm=1)
+    dgrid = 2 * 120e-6 / 126
+    P = np.array([api.finalize_field(fraw[r], dgrid, LAMBDA0, SAMPLE * LAMBDA0)[0][:, 0]
+                  for r in range(nrec)])
+    B = np.array([api.finalize_beam(braw[r], 1, np.full(NSLICE, 3400.0), GAMMA0)[0][:, 25]
+                  for r in range(nrec)])
+    gp = np.load(out / "Field" / "power.npy")
+    gb = np.load(out / "Beam" / "bunching.npy")
+    assert gp.shape == (nrec, NSLICE) and gb.shape == (nrec, NSLICE)
+    ep, eb = relerr(gp[1:], P[1:]), relerr(gb, B)
+    print(f"  power rel err {ep:.3e}  bunching rel err {eb:.3e}", flush=True)
+    assert np.all(gp[-1] > 0)
+    assert ep < 1e-10 and eb < 1e-10
+    z = np.load(out / "Global_zplot.npy").ravel()
+    assert z[0] == 0 and z[-1] == pytest.approx(steps[len(steps) - 1].z)
+
+
+def test_g4run_cli_writes_the_same_records(tmp_path):
+    import subprocess
+    print("case deck cli", flush=True)
+    deck = _deck(tmp_path, "")
+    cli = Path(api.__file__).resolve().parent / "_lib" / "g4run"
+    res = subprocess.run([str(cli), str(deck), "-o", str(tmp_path / "cli")], capture_output=True,
+                         text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    nrec = api.deck_run(deck, tmp_path / "lib")
+    assert f"{nrec} records" in res.stdout
+    for name in ("Field/power", "Beam/bunching", "Beam/energy", "Field/xsize"):
+        a = np.load(tmp_path / "cli" / f"{name}.npy")
+        b = np.load(tmp_path / "lib" / f"{name}.npy")
+        assert relerr(a, b) < 1e-12, name
