@@ -233,11 +233,47 @@ __device__ __forceinline__ void gather(const GatherField& f, int slice, P x, P y
   ri = -scale * cim;
 }
 
+// ---- FP64 phase near a known one ----------------------------------------
+// sin and cos of a0 + d from those of a0: the RK4 stages and the final
+// phase of a step differ from the step's initial phase by |d| << 1, so one
+// full sincos per step and field serves all of them.  Taylor series of sin
+// (to d^13) and cos (to d^14) in Horner form; for |d| <= 1/2 the remainders
+// are below 3e-17, i.e. the result is the direct sincos(a0 + d) to a few
+// ulp.  Larger offsets take the full routine.
+__device__ __forceinline__ void sincos_near(double d, double a, double s0, double c0, double& s,
+                                            double& c) {
+  if (fabs(d) > 0.5) {
+    sincos(a, &s, &c);
+    return;
+  }
+  const double d2 = d * d;
+  double ps = -1.0 / 6227020800.0;  // -1/13!
+  ps = fma(ps, d2, 1.0 / 39916800.0);
+  ps = fma(ps, d2, -1.0 / 362880.0);
+  ps = fma(ps, d2, 1.0 / 5040.0);
+  ps = fma(ps, d2, -1.0 / 120.0);
+  ps = fma(ps, d2, 1.0 / 6.0);
+  const double sd = fma(-ps * d2, d, d);  // d - d^3/3! + ...
+  double pc = 1.0 / 87178291200.0;        // 1/14!
+  pc = fma(pc, d2, -1.0 / 479001600.0);
+  pc = fma(pc, d2, 1.0 / 3628800.0);
+  pc = fma(pc, d2, -1.0 / 40320.0);
+  pc = fma(pc, d2, 1.0 / 720.0);
+  pc = fma(pc, d2, -1.0 / 24.0);
+  pc = fma(pc, d2, 0.5);
+  const double cd = fma(-pc, d2, 1.0);  // 1 - d^2/2! + ...
+  s = fma(s0, cd, c0 * sd);
+  c = fma(c0, cd, -s0 * sd);
+}
+
 // ---- ODE ----------------------------------------------------------------
-// FP64: evaluate_longitudinal_ode_* (BeamSolverCUDA.cpp:145-223) verbatim.
+// FP64: evaluate_longitudinal_ode_* (BeamSolverCUDA.cpp:145-223); the phase
+// factors exp(-i h theta) come from the step's initial ones (bs, bc at
+// theta0 = tthet - dth) through sincos_near.
 template <int NF>
 __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, double tthet,
-                                      double btpar, double ez, const double* rr, const double* ri,
+                                      double dth, const double* bs, const double* bc, double btpar,
+                                      double ez, const double* rr, const double* ri,
                                       double& k2gg, double& k2pp) {
   const double ztemp1 = -2.0 / p.xks;
   double cre = 0.0, cim = 0.0;
@@ -245,7 +281,7 @@ __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, 
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
     double s, c;
-    sincos(p.f[i].rharm * tthet, &s, &c);
+    sincos_near(p.f[i].rharm * dth, p.f[i].rharm * tthet, bs[i], bc[i], s, c);
     cre += rr[i] * c + ri[i] * s;
     cim += ri[i] * c - rr[i] * s;
   }
@@ -284,7 +320,8 @@ __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int 
 // RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
 template <int NF>
 __device__ __forceinline__ void push64(const PushParams& p, int slice, double x, double y, double px,
-                                       double py, double& gamma, double& theta) {
+                                       double py, double& gamma, double& theta, double& base_t,
+                                       double& base_s, double& base_c) {
   const int nf = NF > 0 ? NF : p.nfield;
   double g = gamma;
   double t = theta + p.autophase;
@@ -298,8 +335,16 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, double x,
     if (NF == 0 && i >= nf) break;
     gather<double>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
   }
+  // phase factors at the step's initial phase, one full sincos per field
+  const double t0 = t;
+  double bs[NF > 0 ? NF : kMaxFields], bc[NF > 0 ? NF : kMaxFields];
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    sincos(p.f[i].rharm * t0, &bs[i], &bc[i]);
+  }
   double k2gg = 0.0, k2pp = 0.0, k3gg, k3pp;
-  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<NF>(p, nf, g, t, 0.0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   double stpz = 0.5 * p.delz;
   g += stpz * k2gg;
   t += stpz * k2pp;
@@ -307,14 +352,14 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, double x,
   k3pp = k2pp;
   k2gg = 0.0;
   k2pp = 0.0;
-  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k2gg - k3gg);
   t += stpz * (k2pp - k3pp);
   k3gg /= 6.0;
   k3pp /= 6.0;
   k2gg *= -0.5;
   k2pp *= -0.5;
-  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   stpz = p.delz;
   g += stpz * k2gg;
   t += stpz * k2pp;
@@ -322,11 +367,16 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, double x,
   k3pp -= k2pp;
   k2gg *= 2.0;
   k2pp *= 2.0;
-  ode64<NF>(p, nf, g, t, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k3gg + k2gg / 6.0);
   t += stpz * (k3pp + k2pp / 6.0);
   gamma = g;
   theta = t;
+  // hand the first field's initial phase factors on to the deposition and
+  // the bunching sums (which need the phase after this step)
+  base_t = t0;
+  base_s = bs[0];
+  base_c = bc[0];
 }
 
 template <int NF>
@@ -429,11 +479,16 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   const double g0 = prm.gamma0;
   const int ng = prm.dep.ngrid;
   if (st & ST_T1) transverse<P>(prm.t1, g0, g, x, y, px, py, true);
+  // FP64: sine and cosine of a phase near which the final phase lies
+  // (base_t), known after the push
+  double base_t = 0.0, base_s = 0.0, base_c = 1.0;
+  bool have_base = false;
   if (st & ST_PUSH) {
     if constexpr (sizeof(P) == 8) {
       double gg = g;
-      push64<NF>(prm.push, s, x, y, px, py, gg, th);
+      push64<NF>(prm.push, s, x, y, px, py, gg, th, base_t, base_s, base_c);
       g = gg;
+      have_base = prm.push.f[0].rharm == 1.0;
     } else {
       float gg = g;
       push32<NF>(prm.push, mk, s, x, y, px, py, gg, th);
@@ -459,7 +514,10 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
     if (!have1) {
       if constexpr (sizeof(P) == 8) {
         double a, c;
-        sincos(th, &a, &c);
+        if (have_base)
+          sincos_near(th - base_t, th, base_s, base_c, a, c);
+        else
+          sincos(th, &a, &c);
         sn1 = a;
         cs1 = c;
       } else {
