@@ -69,6 +69,26 @@ def algorithmic_bytes(cls: str, prec: int, np_: int, cells: int) -> float:
     return 0.0
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_fft256_4096x16384_mixed.md"
+TRAFFIC_KERNEL = {"particle": "particle_kernel<float, 1, true, 1>",
+                  "fft_rows": "fft_rows_reg_kernel<float, 256>",
+                  "fft_cols": "fft_cols_reg_kernel<float, 256>"}
+
+
+def measured_traffic(cls: str):
+    """HBM bytes per launch of a MIXED kernel class at the benchmark workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already
+    scaled by the copy-kernel calibration); None when not profiled."""
+    name = TRAFFIC_KERNEL.get(cls)
+    if name is None or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == f"`{name}`":
+            return float(cells[4]) * 1e9
+    return None
+
+
 def run_gpu(prec: int, beam, steps, ngrid: int, gridmax: float, warmup: int, nsteps: int):
     from genesis4_cuda_port_amd import api
     from genesis4_cuda_port_amd._abi import KERNEL_CLASSES
@@ -141,6 +161,63 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int):
                       f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
 
 
+def serving_requests(n: int, seed: int):
+    """The request stream: 70 % steady-state runs (grid 101/151/201/255,
+    4096/8192/16384 particles, linear + quadratic aw taper with coefficients
+    drawn from [0, 2e-3]) and 30 % time-dependent runs (128-2048 slices,
+    grid 101-255, 4096-16384 particles/slice), all on one 100-period
+    undulator module with FODO focusing, ADI field solver, records every 10
+    steps."""
+    from genesis4_cuda_port_amd import api
+    rng = np.random.default_rng(seed)
+    steps = api.lattice_fodo(LAMBDAU, [AW], 100, 46, 3, 1.5, 10)
+    z = np.arange(1, len(steps) + 1) / len(steps)
+    out = []
+    for _ in range(n):
+        if rng.random() < 0.7:
+            a, b = rng.uniform(0.0, 2e-3, 2)
+            out.append(dict(kind=0, nslice=1, ngrid=int(rng.choice([101, 151, 201, 255])),
+                            npart=int(rng.choice([4096, 8192, 16384])), aw=AW * (1 - a * z - b * z * z),
+                            seed=int(rng.integers(1, 1 << 31)), seed_power=1e3))
+        else:
+            out.append(dict(kind=1, nslice=int(rng.integers(128, 2049)),
+                            ngrid=int(rng.integers(101, 256)),
+                            npart=4 * int(rng.integers(1024, 4097)),
+                            seed=int(rng.integers(1, 1 << 31)), seed_power=0.0))
+    return steps, out
+
+
+def serving(prec: int, n: int, seed: int, rate_hz):
+    """Submit the stream to a request loop (200 GB HBM budget, 4 stream
+    lanes): all at once when rate_hz is None (capacity), else with Poisson
+    arrivals at rate_hz.  Returns the server's latency / throughput stats."""
+    from genesis4_cuda_port_amd import api
+    steps, reqs = serving_requests(n, seed)
+    srv = api.Server(0, hbm_budget=200 * 10 ** 9, nstreams=4)
+    rng = np.random.default_rng(seed + 1)
+    t0 = time.perf_counter()
+    t_next = 0.0
+    ids = []
+    for r in reqs:
+        if rate_hz:
+            t_next += rng.exponential(1.0 / rate_hz)
+            dt = t0 + t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        ids.append(srv.submit(precision=prec, solver=api.ADI, steps=steps, gridmax=0.3e-3,
+                              current=3000.0, gamma0=GAMMA0, lambda0=LAMBDA0, espread_rel=1e-4,
+                              **r))
+    for i in ids:
+        srv.wait(i)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"p50_ms": st["p50_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
+            "mean_ms": st["mean_s"] * 1e3, "requests": int(st["completed"]),
+            "particle_slice_steps_per_s": st["particle_slice_steps_per_s"],
+            "requests_per_s": len(reqs) / wall}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +227,8 @@ def main() -> int:
     ap.add_argument("--particles", type=int, default=16384)
     ap.add_argument("--ngrid", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1536)
     ap.add_argument("--cpu-steps", type=int, default=12)
@@ -210,7 +289,7 @@ def main() -> int:
     achieved = ab / (st["avg_ms"] * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "kernel": name, "achieved": achieved,
                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                       "traffic": None, "algorithmic_bytes_per_launch": ab,
+                       "traffic": measured_traffic(name), "algorithmic_bytes_per_launch": ab,
                        "avg_launch_ms": st["avg_ms"]}
     out["kernels_mixed"] = {k: dict(v, gbs=algorithmic_bytes(k, api.MIXED, np_total, cells) /
                                     (v["avg_ms"] * 1e-3) / 1e9) for k, v in km.items()}
@@ -237,6 +316,22 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
                                            a.cpu_steps)
     out["latency_p50_p99"] = None
+    if not a.no_serve:
+        # capacity with every request queued at once, then Poisson arrivals at
+        # 80 % of that capacity; MIXED and FP64
+        serve = {"workload": "request stream of serving_requests(): 70 % steady-state "
+                             "(batched), 30 % time-dependent; 100-step lattice; ADI; "
+                             "200 GB HBM budget, 4 stream lanes", "seed": 1234571}
+        for name, prec in (("mixed", api.MIXED), ("fp64", api.FP64)):
+            log(f"serving capacity ({name})")
+            cap = serving(prec, a.serve_requests, 1234571, None)
+            rate = 0.8 * cap["requests_per_s"]
+            log(f"serving at {rate:.2f} req/s ({name})")
+            res = serving(prec, a.serve_requests, 1234571, rate)
+            res["rate_hz"] = rate
+            res["capacity_requests_per_s"] = cap["requests_per_s"]
+            serve[name] = res
+        out["latency_p50_p99"] = serve
     print(json.dumps(out), flush=True)
     return 0
 

@@ -80,6 +80,42 @@ class BeamSpec(C.Structure):
     ]
 
 
+class BatchRun(C.Structure):
+    _fields_ = [
+        ("npart", C.c_int32), ("nbins", C.c_int32), ("ngrid", C.c_int32),
+        ("shotnoise", C.c_int32), ("seed", C.c_uint64), ("gridmax", C.c_double),
+        ("current", C.c_double), ("gamma0", C.c_double), ("espread_rel", C.c_double),
+        ("emitn_x", C.c_double), ("emitn_y", C.c_double), ("betax", C.c_double),
+        ("betay", C.c_double), ("alphax", C.c_double), ("alphay", C.c_double),
+        ("seed_power", C.c_double), ("seed_w0", C.c_double),
+        ("aw", C.POINTER(C.c_double)),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("precision", C.c_int32), ("solver", C.c_int32),
+        ("nslice", C.c_int32), ("npart", C.c_int32), ("nbins", C.c_int32), ("ngrid", C.c_int32),
+        ("shotnoise", C.c_int32), ("nharm", C.c_int32), ("periodic", C.c_int32),
+        ("nsteps", C.c_int32), ("seed", C.c_uint64), ("gridmax", C.c_double),
+        ("current", C.c_double), ("gamma0", C.c_double), ("espread_rel", C.c_double),
+        ("chirp_rel", C.c_double), ("emitn", C.c_double), ("beta", C.c_double),
+        ("seed_power", C.c_double), ("seed_w0", C.c_double), ("lambda0", C.c_double),
+        ("sample", C.c_double), ("steps", C.POINTER(Step)), ("aw", C.POINTER(C.c_double)),
+    ]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("nrecords", C.c_int32), ("latency_s", C.c_double),
+                ("queued_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("p50_s", C.c_double), ("p99_s", C.c_double),
+                ("mean_s", C.c_double), ("particle_slice_steps_per_s", C.c_double),
+                ("hbm_in_use", C.c_uint64)]
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _vp = C.c_void_p
@@ -129,6 +165,20 @@ PROTOTYPES = {
     "g4h_run_advance": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "g4h_run_records": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "g4h_run_download_diag": (C.c_int, [_vp, _dp, _dp]),
+    "g4h_batch_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(BatchRun),
+                                   C.POINTER(Step), C.c_int, C.c_double, C.c_int,
+                                   C.POINTER(_vp)]),
+    "g4h_batch_destroy": (None, [_vp]),
+    "g4h_batch_advance": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "g4h_batch_records": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "g4h_batch_download": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "g4h_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(_vp)]),
+    "g4h_server_destroy": (None, [_vp]),
+    "g4h_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "g4h_server_wait": (C.c_int, [_vp, C.c_int64, C.c_double, C.POINTER(ResultInfo)]),
+    "g4h_server_result": (C.c_int, [_vp, C.c_int64, _dp, _dp]),
+    "g4h_server_release": (C.c_int, [_vp, C.c_int64]),
+    "g4h_server_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "g4h_profile_enable": (C.c_int, [_vp, C.c_int]),
     "g4h_profile_read": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), _dp]),
     "g4h_undulator_fc": (C.c_double, [C.c_double, C.c_int, C.c_int]),

@@ -275,6 +275,145 @@ class Run:
         return (braw.reshape(n, S, beam_raw_len(self.nharm)), fraw.reshape(n, S, FIELD_RAW))
 
 
+class Batch:
+    """Independent steady-state runs advanced together (g4h_batch_*).
+
+    runs: list of dicts with keys npart, nbins, ngrid, gridmax, current,
+    gamma0, espread_rel, emitn (or emitn_x/emitn_y), beta (or betax/betay),
+    alphax, alphay, shotnoise, seed, seed_power, seed_w0 and optionally aw
+    (per-step taper, one value per step)."""
+
+    def __init__(self, ctx: Context, precision: int, solver: int, runs, steps, lambda0: float,
+                 nharm: int = 1):
+        from ._abi import BatchRun
+        self.steps = steps if isinstance(steps, C.Array) else steps_array(steps)
+        ns = len(self.steps)
+        arr = (BatchRun * len(runs))()
+        self._aw = []
+        for i, r in enumerate(runs):
+            q = arr[i]
+            q.npart, q.nbins, q.ngrid = r["npart"], r.get("nbins", 4), r["ngrid"]
+            q.shotnoise, q.seed = int(r.get("shotnoise", True)), int(r.get("seed", 1))
+            q.gridmax, q.current, q.gamma0 = r["gridmax"], r["current"], r["gamma0"]
+            q.espread_rel = r.get("espread_rel", 0.0)
+            q.emitn_x = r.get("emitn_x", r.get("emitn", 0.4e-6))
+            q.emitn_y = r.get("emitn_y", r.get("emitn", 0.4e-6))
+            q.betax = r.get("betax", r.get("beta", 15.0))
+            q.betay = r.get("betay", r.get("beta", 15.0))
+            q.alphax, q.alphay = r.get("alphax", 0.0), r.get("alphay", 0.0)
+            q.seed_power, q.seed_w0 = r.get("seed_power", 0.0), r.get("seed_w0", 50e-6)
+            if r.get("aw") is not None:
+                a = _f64(r["aw"], ns)
+                self._aw.append(a)
+                q.aw = _dp(a)
+        h = C.c_void_p()
+        check(lib().g4h_batch_create(ctx.h, precision, solver, len(runs), arr, self.steps, ns,
+                                     lambda0, nharm, C.byref(h)))
+        self.h, self.ctx, self.nharm, self.nruns = h, ctx, nharm, len(runs)
+
+    def close(self):
+        if self.h:
+            lib().g4h_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, nsteps: int) -> int:
+        done = C.c_int()
+        check(lib().g4h_batch_advance(self.h, nsteps, C.byref(done)))
+        return int(done.value)
+
+    def records(self) -> int:
+        n = C.c_int()
+        check(lib().g4h_batch_records(self.h, C.byref(n)))
+        return int(n.value)
+
+    def download(self, run: int):
+        """(beam_raw[records, RAW], field_raw[records, 9]) of one run."""
+        n = self.records()
+        braw = np.empty(n * beam_raw_len(self.nharm))
+        fraw = np.empty(n * FIELD_RAW)
+        check(lib().g4h_batch_download(self.h, run, _dp(braw), _dp(fraw)))
+        return braw.reshape(n, beam_raw_len(self.nharm)), fraw.reshape(n, FIELD_RAW)
+
+
+class Server:
+    """Request loop keeping many runs resident on one GPU (g4h_server_*)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 200 << 30, nstreams: int = 4):
+        h = C.c_void_p()
+        check(lib().g4h_server_create(device, hbm_budget, nstreams, C.byref(h)))
+        self.h = h
+        self._meta = {}
+
+    def close(self):
+        if self.h:
+            lib().g4h_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, *, kind: int, precision: int, solver: int, steps, nslice: int, npart: int,
+               ngrid: int, gridmax: float, current: float, gamma0: float, lambda0: float,
+               espread_rel: float = 0.0, chirp_rel: float = 0.0, emitn: float = 0.4e-6,
+               beta: float = 15.0, nbins: int = 4, shotnoise: bool = True, seed: int = 1,
+               seed_power: float = 0.0, seed_w0: float = 50e-6, sample: float = 1.0,
+               nharm: int = 1, periodic: bool = False, aw=None) -> int:
+        from ._abi import Request
+        st = steps if isinstance(steps, C.Array) else steps_array(steps)
+        q = Request()
+        q.kind, q.precision, q.solver = kind, precision, solver
+        q.nslice, q.npart, q.nbins, q.ngrid = nslice, npart, nbins, ngrid
+        q.shotnoise, q.nharm, q.periodic, q.nsteps = int(shotnoise), nharm, int(periodic), len(st)
+        q.seed, q.gridmax, q.current, q.gamma0 = seed, gridmax, current, gamma0
+        q.espread_rel, q.chirp_rel, q.emitn, q.beta = espread_rel, chirp_rel, emitn, beta
+        q.seed_power, q.seed_w0, q.lambda0, q.sample = seed_power, seed_w0, lambda0, sample
+        q.steps = st
+        a = None
+        if aw is not None:
+            a = _f64(aw, len(st))
+            q.aw = _dp(a)
+        i = C.c_int64()
+        check(lib().g4h_server_submit(self.h, C.byref(q), C.byref(i)))
+        self._meta[int(i.value)] = (nslice, nharm)
+        return int(i.value)
+
+    def wait(self, rid: int, timeout: float = -1.0):
+        from ._abi import ResultInfo
+        info = ResultInfo()
+        check(lib().g4h_server_wait(self.h, rid, timeout, C.byref(info)))
+        return info
+
+    def result(self, rid: int):
+        info = self.wait(rid)
+        S, nh = self._meta[rid]
+        n = info.nrecords
+        braw = np.empty(n * S * beam_raw_len(nh))
+        fraw = np.empty(n * S * FIELD_RAW)
+        check(lib().g4h_server_result(self.h, rid, _dp(braw), _dp(fraw)))
+        return braw.reshape(n, S, beam_raw_len(nh)), fraw.reshape(n, S, FIELD_RAW)
+
+    def release(self, rid: int):
+        check(lib().g4h_server_release(self.h, rid))
+        self._meta.pop(rid, None)
+
+    def stats(self):
+        from ._abi import ServerStats
+        st = ServerStats()
+        check(lib().g4h_server_stats(self.h, C.byref(st)))
+        return {"completed": st.completed, "p50_s": st.p50_s, "p99_s": st.p99_s,
+                "mean_s": st.mean_s, "particle_slice_steps_per_s": st.particle_slice_steps_per_s,
+                "hbm_in_use": st.hbm_in_use}
+
+
 # ---- front-end --------------------------------------------------------------
 
 def undulator_fc(aw: float, helical: bool, harm: int) -> float:

@@ -29,6 +29,8 @@ SOURCES = [
     "field_misc.hip",
     "abi.cpp",
     "run.cpp",
+    "batch.cpp",
+    "serve.cpp",
     "frontend.cpp",
     "deck.cpp",
     "deck_run.cpp",

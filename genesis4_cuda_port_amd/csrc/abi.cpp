@@ -110,6 +110,12 @@ TransParams make_trans(const g4h_step& st, double gamma0, double delz, bool last
   t.delz = delz;
   t.aw2 = aw * aw;
   t.g2c = gamma0 * gamma0 - 1.0 - aw * aw;
+  {
+    // K = str/gammaz with gammaz >= gammaz(gamma0)/2 for gamma >= gamma0/2
+    const double gz_min = 0.5 * std::sqrt(std::max(t.g2c, 1.0));
+    t.xpoly = t.xmode != 0 && std::fabs(t.xstr) / gz_min * delz * delz <= 0.5 ? 1 : 0;
+    t.ypoly = t.ymode != 0 && std::fabs(t.ystr) / gz_min * delz * delz <= 0.5 ? 1 : 0;
+  }
   t.g2chic = gamma0 * gamma0 - 1.0;
   if (last) {
     t.corrector = (st.cx != 0 || st.cy != 0) ? 1 : 0;

@@ -1,0 +1,318 @@
+// batch.cpp - many independent steady-state runs advanced together.
+//
+// A taper scan or a stream of small requests is many one-slice runs that
+// share a lattice but differ in particle count, grid and undulator taper.
+// Launched one by one they leave the GPU idle (one workgroup per run); here
+// runs with the same grid (ngrid, gridmax) and reference energy form a group
+// that is one Beam (one slice per run, uneven particle counts through the
+// slice offsets) and one Field (one ring plane per run), so each step of the
+// whole group is one particle launch, one solver step and, on output steps,
+// one field-sum map.  Per-run step parameters (the taper changes aw, and with
+// it the focusing, the coupling fc(aw) and the phase-equation constants)
+// reach the particle pass through a per-slice parameter table, built on the
+// host from the same make_trans / make_push / make_dep as every other path
+// and copied through a pinned ring of tables.
+//
+// Steady state only: one slice per run, no slippage (Gencore.cpp:229 is a
+// no-op there, ControlCUDA.cpp:150).
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace g4 {
+void context_child_added(Context* c);
+void context_child_removed(Context* c);
+void set_last_error(const std::string& m);
+}  // namespace g4
+
+using namespace g4;
+
+namespace {
+
+constexpr int kRing = 8;  // parameter tables in flight per group
+
+void ok(g4h_status s) {
+  if (s != G4H_OK) throw Error(s, g4h_last_error());
+}
+
+struct Group {
+  int G = 0;
+  double gridmax = 0, gamma0 = 0;
+  std::vector<int> runs;
+  g4h_beam* beam = nullptr;
+  g4h_field* field = nullptr;
+  DevBuf<ParticleParams> dtab;       // kRing tables of runs.size() entries
+  ParticleParams* htab = nullptr;    // pinned host copy of the ring
+  hipEvent_t used[kRing] = {};       // last launch that read each ring slot
+  bool pending[kRing] = {};
+  DevBuf<double> beam_hist, field_hist;
+};
+
+}  // namespace
+
+struct g4h_batch {
+  Context* ctx = nullptr;
+  g4h_precision prec = G4H_FP64;
+  g4h_solver_kind solver = G4H_SOLVER_ADI;
+  double lambda0 = 0;
+  int nharm = 1;
+  std::vector<g4h_step> steps;
+  std::vector<std::vector<double>> aw;  // per run, per step (empty: the step table's)
+  std::vector<Group> groups;
+  std::vector<std::pair<int, int>> where;  // run -> (group, slice in group)
+  int done = 0, nrec = 0, cap = 0;
+  int slot = 0;
+
+  ~g4h_batch() {
+    for (Group& g : groups) {
+      if (g.field) g4h_field_destroy(g.field);
+      if (g.beam) g4h_beam_destroy(g.beam);
+      for (hipEvent_t e : g.used)
+        if (e) (void)hipEventDestroy(e);
+      if (g.htab) (void)hipHostFree(g.htab);
+    }
+  }
+
+  size_t beam_rec(const Group& g) const {
+    return g.runs.size() * static_cast<size_t>(G4H_BEAM_RAW(nharm));
+  }
+  size_t field_rec(const Group& g) const { return g.runs.size() * G4H_FIELD_RAW; }
+
+  void record_initial() {
+    for (Group& g : groups) {
+      ParticleParams p{};
+      p.gamma0 = g.gamma0;
+      p.stages = ST_DIAG;
+      p.diag.raw = g.beam_hist.get();
+      p.diag.nharm = nharm;
+      launch_particle(ctx, g.beam, p);
+      field_plane_stats(g.field);
+      field_map_stats(g.field, g.field_hist.get());
+    }
+    nrec = 1;
+  }
+
+  void step(int istep) {
+    const g4h_step& st0 = steps[static_cast<size_t>(istep)];
+    const double delz = st0.delz;
+    const bool out = st0.outstep && nrec < cap;
+    const int sl = slot;
+    slot = (slot + 1) % kRing;
+    for (Group& g : groups) {
+      const int n = static_cast<int>(g.runs.size());
+      if (g.pending[sl]) G4_HIP(hipEventSynchronize(g.used[sl]));
+      field_prepare_solver(g.field, delz);
+      ParticleParams* tab = g.htab + static_cast<size_t>(sl) * n;
+      bool src_on = false;
+      for (int i = 0; i < n; ++i) {
+        g4h_step st = st0;
+        const auto& a = aw[static_cast<size_t>(g.runs[static_cast<size_t>(i)])];
+        if (!a.empty()) st.aw = a[static_cast<size_t>(istep)];
+        ParticleParams p{};
+        p.gamma0 = g.gamma0;
+        p.stages = ST_T1 | ST_PUSH | ST_T2;
+        p.t1 = make_trans(st, g.gamma0, delz * 0.5, false);
+        p.t2 = make_trans(st, g.gamma0, delz * 0.5, true);
+        Field* fl[1] = {g.field};
+        make_push(p.push, st, g.gamma0, delz, fl, 1, nullptr);
+        p.r56 = make_r56(st, g.gamma0, lambda0);
+        if (p.r56 != 0.0) p.stages |= ST_R56;
+        const bool on = make_dep(p.dep, g.field, g.beam, st, delz) && g.beam->np > 0;
+        require(i == 0 || on == src_on, "source switched on for part of a batch group");
+        src_on = on;
+        if (on) p.stages |= ST_DEP;
+        if (out) {
+          p.stages |= ST_DIAG;
+          p.diag.raw = g.beam_hist.get() + static_cast<size_t>(nrec) * beam_rec(g);
+          p.diag.nharm = nharm;
+        }
+        tab[i] = p;
+      }
+      ParticleParams* dtab = g.dtab.get() + static_cast<size_t>(sl) * n;
+      G4_HIP(hipMemcpyAsync(dtab, tab, sizeof(ParticleParams) * n, hipMemcpyHostToDevice,
+                            ctx->stream));
+      launch_particle(ctx, g.beam, tab[0], dtab);
+      G4_HIP(hipEventRecord(g.used[sl], ctx->stream));
+      g.pending[sl] = true;
+      field_solve(g.field, src_on, out);
+      if (out) field_map_stats(g.field, g.field_hist.get() + static_cast<size_t>(nrec) * field_rec(g));
+    }
+    if (out) ++nrec;
+  }
+};
+
+#define G4_API_BEGIN try {
+#define G4_API_END                     \
+  return G4H_OK;                       \
+  }                                    \
+  catch (const g4::Error& e) {         \
+    g4::set_last_error(e.what());      \
+    return e.code;                     \
+  }                                    \
+  catch (const std::exception& e) {    \
+    g4::set_last_error(e.what());      \
+    return G4H_ERR_DEVICE;             \
+  }
+
+extern "C" {
+
+g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solver_kind solver,
+                            int nruns, const g4h_batch_run* runs, const g4h_step* steps,
+                            int nsteps, double lambda0, int nharm, g4h_batch** out) {
+  G4_API_BEGIN
+  require(ctx && runs && steps && out, "null argument");
+  *out = nullptr;
+  require(nruns >= 1 && nsteps >= 1, "need at least one run and one step");
+  require(nharm >= 1 && nharm <= 8, "nharm must be in [1, 8]");
+  require(lambda0 > 0, "lambda0 must be positive");
+  require(precision == G4H_FP64 || precision == G4H_MIXED, "bad precision");
+  require(solver == G4H_SOLVER_ADI || solver == G4H_SOLVER_FFT, "bad solver");
+  for (int i = 0; i < nsteps; ++i) require(steps[i].delz > 0, "step length must be positive");
+  ctx->activate();
+  auto b = std::make_unique<g4h_batch>();
+  b->ctx = ctx;
+  b->prec = precision;
+  b->solver = solver;
+  b->lambda0 = lambda0;
+  b->nharm = nharm;
+  b->steps.assign(steps, steps + nsteps);
+  int nout = 0;
+  for (int i = 0; i < nsteps; ++i) nout += steps[i].outstep ? 1 : 0;
+  b->cap = 1 + nout;
+  // group by (ngrid, gridmax, gamma0), runs in request order within a group
+  std::map<std::tuple<int, double, double>, int> key;
+  b->where.resize(static_cast<size_t>(nruns));
+  b->aw.resize(static_cast<size_t>(nruns));
+  for (int r = 0; r < nruns; ++r) {
+    const g4h_batch_run& q = runs[r];
+    require(q.npart >= 1 && q.nbins >= 1 && q.npart % q.nbins == 0,
+            "npart must be a positive multiple of nbins");
+    require(q.ngrid >= 3 && q.ngrid <= 1024 && q.gridmax > 0, "bad grid");
+    require(q.gamma0 > 1 && q.current >= 0, "bad beam");
+    const auto k = std::make_tuple(q.ngrid, q.gridmax, q.gamma0);
+    auto it = key.find(k);
+    if (it == key.end()) {
+      it = key.emplace(k, static_cast<int>(b->groups.size())).first;
+      Group g;
+      g.G = q.ngrid;
+      g.gridmax = q.gridmax;
+      g.gamma0 = q.gamma0;
+      b->groups.push_back(std::move(g));
+    }
+    Group& g = b->groups[static_cast<size_t>(it->second)];
+    b->where[static_cast<size_t>(r)] = {it->second, static_cast<int>(g.runs.size())};
+    g.runs.push_back(r);
+    if (q.aw) b->aw[static_cast<size_t>(r)].assign(q.aw, q.aw + nsteps);
+  }
+  for (Group& g : b->groups) {
+    const int n = static_cast<int>(g.runs.size());
+    std::vector<int32_t> off(static_cast<size_t>(n) + 1, 0);
+    std::vector<double> cur(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+      const g4h_batch_run& q = runs[g.runs[static_cast<size_t>(i)]];
+      off[static_cast<size_t>(i) + 1] = off[static_cast<size_t>(i)] + q.npart;
+      cur[static_cast<size_t>(i)] = q.current;
+    }
+    const size_t np = static_cast<size_t>(off.back());
+    std::vector<double> x(np), y(np), px(np), py(np), ga(np), th(np);
+    std::vector<double> re(static_cast<size_t>(n) * g.G * g.G), im(re.size());
+    for (int i = 0; i < n; ++i) {
+      const g4h_batch_run& q = runs[g.runs[static_cast<size_t>(i)]];
+      g4h_beam_spec s{};
+      s.gamma0 = q.gamma0;
+      s.espread_rel = q.espread_rel;
+      s.emitn_x = q.emitn_x;
+      s.emitn_y = q.emitn_y;
+      s.betax = q.betax;
+      s.betay = q.betay;
+      s.alphax = q.alphax;
+      s.alphay = q.alphay;
+      s.slicelength = lambda0;
+      s.nslice = 1;
+      s.npart = q.npart;
+      s.nbins = q.nbins;
+      s.shotnoise = q.shotnoise;
+      s.seed = q.seed;
+      const size_t o = static_cast<size_t>(off[static_cast<size_t>(i)]);
+      ok(g4h_load_beam(&s, &q.current, x.data() + o, y.data() + o, px.data() + o, py.data() + o,
+                       ga.data() + o, th.data() + o));
+      if (q.seed_power > 0) {
+        const size_t po = static_cast<size_t>(i) * g.G * g.G;
+        ok(g4h_seed_gaussian(1, g.G, g.gridmax, lambda0, q.seed_power, q.seed_w0, re.data() + po,
+                             im.data() + po));
+      }
+    }
+    ok(g4h_beam_create(ctx, precision, n, off.data(), cur.data(), g.gamma0, &g.beam));
+    ok(g4h_beam_upload(g.beam, x.data(), y.data(), px.data(), py.data(), ga.data(), th.data()));
+    ok(g4h_field_create(ctx, precision, n, g.G, g.gridmax, lambda0, lambda0, 1, &g.field));
+    ok(g4h_field_upload(g.field, re.data(), im.data(), 0));
+    ok(g4h_field_init_solver(g.field, solver, 0, 1.0, 1.0, 1.0, 0));
+    g.dtab.alloc(ctx, static_cast<size_t>(kRing) * n);
+    void* h = nullptr;
+    G4_HIP(hipHostMalloc(&h, sizeof(ParticleParams) * kRing * n, hipHostMallocDefault));
+    g.htab = static_cast<ParticleParams*>(h);
+    for (hipEvent_t& e : g.used) G4_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g.beam_hist.alloc(ctx, static_cast<size_t>(b->cap) * b->beam_rec(g));
+    g.field_hist.alloc(ctx, static_cast<size_t>(b->cap) * b->field_rec(g));
+  }
+  b->record_initial();
+  *out = b.release();
+  context_child_added(ctx);
+  G4_API_END
+}
+
+void g4h_batch_destroy(g4h_batch* b) {
+  if (!b) return;
+  Context* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete b;
+  context_child_removed(ctx);
+}
+
+g4h_status g4h_batch_advance(g4h_batch* b, int nsteps, int* done) {
+  G4_API_BEGIN
+  require(b, "batch is null");
+  require(nsteps >= 0, "nsteps must be >= 0");
+  b->ctx->activate();
+  const int end = std::min<int>(static_cast<int>(b->steps.size()), b->done + nsteps);
+  for (; b->done < end; ++b->done) b->step(b->done);
+  if (done) *done = b->done;
+  G4_API_END
+}
+
+g4h_status g4h_batch_records(const g4h_batch* b, int* nrecords) {
+  G4_API_BEGIN
+  require(b && nrecords, "null argument");
+  *nrecords = b->nrec;
+  G4_API_END
+}
+
+g4h_status g4h_batch_download(g4h_batch* b, int run, double* beam_raw, double* field_raw) {
+  G4_API_BEGIN
+  require(b, "batch is null");
+  require(run >= 0 && run < static_cast<int>(b->where.size()), "run out of range");
+  b->ctx->activate();
+  G4_HIP(hipStreamSynchronize(b->ctx->stream));
+  const auto [gi, si] = b->where[static_cast<size_t>(run)];
+  const Group& g = b->groups[static_cast<size_t>(gi)];
+  const size_t br = G4H_BEAM_RAW(b->nharm);
+  for (int r = 0; r < b->nrec; ++r) {
+    if (beam_raw)
+      G4_HIP(hipMemcpy(beam_raw + static_cast<size_t>(r) * br,
+                       g.beam_hist.get() + static_cast<size_t>(r) * b->beam_rec(g) + si * br,
+                       sizeof(double) * br, hipMemcpyDeviceToHost));
+    if (field_raw)
+      G4_HIP(hipMemcpy(field_raw + static_cast<size_t>(r) * G4H_FIELD_RAW,
+                       g.field_hist.get() + static_cast<size_t>(r) * b->field_rec(g) +
+                           static_cast<size_t>(si) * G4H_FIELD_RAW,
+                       sizeof(double) * G4H_FIELD_RAW, hipMemcpyDeviceToHost));
+  }
+  G4_API_END
+}
+
+}  // extern "C"

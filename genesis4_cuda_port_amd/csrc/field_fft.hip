@@ -331,18 +331,32 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
   T2* lbuf = buf + (wave * S::LPW + g) * S::BUF;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * C::kWaves;
   const P sc = static_cast<P>(scale);
-  for (int64_t grp = blockIdx.x * static_cast<int64_t>(C::kWaves) + wave; grp * S::LPW < nrows;
-       grp += nwaves) {
+  // software pipelined: the next line group is loaded before this one is
+  // transformed (invalid line groups compute on zeros)
+  auto load = [&](int64_t grp, T2 (&d)[R]) {
     const int64_t line = grp * S::LPW + g;
-    const bool valid = line < nrows;  // invalid line groups compute on zeros
+    const bool ok = line < nrows;
+    const T2* row = u + line * G;
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      d[r].x = 0;
+      d[r].y = 0;
+      if (ok) d[r] = row[t + L * r];
+    });
+  };
+  int64_t grp = blockIdx.x * static_cast<int64_t>(C::kWaves) + wave;
+  T2 vn[R];
+  if (grp * S::LPW < nrows) load(grp, vn);
+  for (; grp * S::LPW < nrows; grp += nwaves) {
+    const int64_t line = grp * S::LPW + g;
+    const bool valid = line < nrows;
     T2* row = u + line * G;
     T2 v[R];
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
-      v[r].x = 0;
-      v[r].y = 0;
-      if (valid) v[r] = row[t + L * r];
+      v[r] = vn[r];
     });
+    if ((grp + nwaves) * S::LPW < nrows) load(grp + nwaves, vn);
     if (mode == ROW_PROP || mode == ROW_FWD) fftr::forward<T2, G>(v, t, tw, lbuf);
     if (mode == ROW_PROP) {
       fftr::static_for<0, R>([&](auto ic) {

@@ -26,6 +26,9 @@ struct TransParams {
   double g2chic;   // gamma0^2 - 1          (mixed chicane gammaz)
   int xmode, ymode;  // 0 drift, 1 focusing, 2 defocusing
   int chicane, corrector;
+  // FP32 may use the power series of the plane maps in K dz^2 (|K dz^2|
+  // <= 1/2 for every particle with gamma >= gamma0/2)
+  int xpoly, ypoly;
 };
 
 struct GatherField {
@@ -73,7 +76,11 @@ struct ParticleParams {
   DiagParams diag;
 };
 
-void launch_particle(Context* ctx, Beam* b, const ParticleParams& p);
+// With d_table (device, one entry per slice) each slice's workgroup takes its
+// parameters from the table; p still selects the kernel variant (stages,
+// field count, harmonics, window size), which must be common to all entries.
+void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
+                     const ParticleParams* d_table = nullptr);
 void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell);
 
 // Host construction of parameter blocks from a lattice step.

@@ -100,12 +100,35 @@ __device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& co
 }
 
 // MIXED overload: fast reciprocal / square root and hardware sin/cos (the
-// focusing phase advance per half step is ~1e-2 rad).
+// focusing phase advance per half step is ~1e-2 rad).  With `poly` the map
+// is taken from its power series in w = K dz^2 (K = str/gammaz, either
+// sign: cos(sqrt(w)) and sin(sqrt(w))/sqrt(w) are entire in w and cover
+// focusing and defocusing alike), to w^4: for |w| <= 1/2 the remainders are
+// below 1e-8, under FP32 rounding, and no transcendental is evaluated.
 __device__ __forceinline__ void plane_step(int mode, float str, float off, float delz,
-                                           float& coord, float& mom, float gz) {
-  const float igz = __builtin_amdgcn_rcpf(gz);
+                                           float& coord, float& mom, float gz, float igz,
+                                           bool poly) {
   if (mode == 0) {
     coord += mom * delz * igz;
+    return;
+  }
+  if (poly) {
+    const float k = str * igz;
+    const float w = k * delz * delz;
+    float c = 1.0f / 40320.0f;  // cos(sqrt(w)) = 1 - w/2 + w^2/24 - ...
+    c = fmaf(c, w, -1.0f / 720.0f);
+    c = fmaf(c, w, 1.0f / 24.0f);
+    c = fmaf(c, w, -0.5f);
+    c = fmaf(c, w, 1.0f);
+    float sn = 1.0f / 362880.0f;  // sin(sqrt(w))/sqrt(w) = 1 - w/6 + w^2/120 - ...
+    sn = fmaf(sn, w, -1.0f / 5040.0f);
+    sn = fmaf(sn, w, 1.0f / 120.0f);
+    sn = fmaf(sn, w, -1.0f / 6.0f);
+    sn = fmaf(sn, w, 1.0f);
+    const float a2 = delz * sn;  // sin(omega)/foc
+    const float xt = coord - off;
+    coord = fmaf(c, xt, a2 * mom * igz) + off;
+    mom = fmaf(-k * a2 * gz, xt, c * mom);
     return;
   }
   if (mode == 1) {
@@ -162,9 +185,19 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
     px += P(t.cx);
     py += P(t.cy);
   }
-  const P gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
-  plane_step(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
-  plane_step(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
+  if constexpr (sizeof(P) == 8) {
+    const P gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
+    plane_step(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
+    plane_step(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
+  } else {
+    const float gzz = gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py;
+    const float igz = __builtin_amdgcn_rsqf(gzz);  // 1/gammaz
+    const float gz = gzz * igz;
+    plane_step(t.xmode, float(t.xstr), float(t.xoff), float(t.delz), x, px, gz, igz,
+               t.xpoly != 0);
+    plane_step(t.ymode, float(t.ystr), float(t.yoff), float(t.delz), y, py, gz, igz,
+               t.ypoly != 0);
+  }
 }
 
 // ---- particle -> cell (BeamSolverCUDA.cpp:99-116, FieldSolverADICUDA.cpp:
@@ -553,22 +586,34 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
         const P cre = sn * part;
         const P cim = cs * part;
         const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
+        const int lx0 = ix - wx0, ly0 = iy - wy0;
+        if (lx0 >= 0 && lx0 + 1 < ww && ly0 >= 0 && ly0 + 1 < wh) {
+          // the whole 2x2 block lies in the window (the common case)
+          double* a = &win[lx0 + ly0 * ww].x;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int cx = ix + (k & 1), cy = iy + (k >> 1);
-          const int lx = cx - wx0, ly = cy - wy0;
-          if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
-            double* a = &win[lx + ly * ww].x;
-            lds_add(a, static_cast<double>(cre * w[k]));
-            lds_add(a + 1, static_cast<double>(cim * w[k]));
-          } else {
-            P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
-            glob_add(a, cre * w[k]);
-            glob_add(a + 1, cim * w[k]);
-            atomicMin(&s_fb[0], cx);
-            atomicMin(&s_fb[1], cy);
-            atomicMax(&s_fb[2], cx);
-            atomicMax(&s_fb[3], cy);
+          for (int k = 0; k < 4; ++k) {
+            double* ak = a + 2 * ((k & 1) + (k >> 1) * ww);
+            lds_add(ak, static_cast<double>(cre * w[k]));
+            lds_add(ak + 1, static_cast<double>(cim * w[k]));
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int cx = ix + (k & 1), cy = iy + (k >> 1);
+            const int lx = cx - wx0, ly = cy - wy0;
+            if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+              double* a = &win[lx + ly * ww].x;
+              lds_add(a, static_cast<double>(cre * w[k]));
+              lds_add(a + 1, static_cast<double>(cim * w[k]));
+            } else {
+              P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
+              glob_add(a, cre * w[k]);
+              glob_add(a + 1, cim * w[k]);
+              atomicMin(&s_fb[0], cx);
+              atomicMin(&s_fb[1], cy);
+              atomicMax(&s_fb[2], cx);
+              atomicMax(&s_fb[3], cy);
+            }
           }
         }
       }
@@ -624,9 +669,23 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   }
 }
 
-template <class P, int NF, bool DEP, int NH>
+// Parameters of this workgroup's slice: the launch's by-value block, or
+// with TAB the slice's entry of a device table (batches of independent runs
+// whose per-step parameters differ from run to run).
+template <bool TAB>
+__device__ __forceinline__ const ParticleParams& slice_params(const ParticleParams& a,
+                                                             const ParticleParams* tab) {
+  if constexpr (TAB)
+    return tab[blockIdx.x];
+  else
+    return a;
+}
+
+template <class P, int NF, bool DEP, int NH, bool TAB>
 __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
-                                                            ParticleParams prm) {
+                                                            ParticleParams prm_arg,
+                                                            const ParticleParams* __restrict__ tab) {
+  const ParticleParams& prm = slice_params<TAB>(prm_arg, tab);
   using T2 = typename V2<P>::T;
   constexpr int NSUM = NH > 0 ? 12 + 2 * NH : 1;  // sums other than the count
   extern __shared__ double2 win[];
@@ -881,35 +940,43 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
                 : -1;
 }
 
-template <class P, int NF, bool DEP, int NH>
-void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
+template <class P, int NF, bool DEP, int NH, bool TAB>
+void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
+              const ParticleParams* tab) {
   const size_t lds = DEP ? static_cast<size_t>(p.dep.window_cap) * sizeof(double2) : 0;
   ctx->launch(G4H_K_PARTICLE, [&] {
-    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH>), dim3(b->nslice), dim3(kThreads), lds,
-                       ctx->stream, bp, b->off.get(), p);
+    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH, TAB>), dim3(b->nslice), dim3(kThreads),
+                       lds, ctx->stream, bp, b->off.get(), p, tab);
   });
 }
 
-template <class P, int NF, bool DEP>
-void dispatch_nh(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
-  if (!(p.stages & ST_DIAG)) return launch_t<P, NF, DEP, 0>(ctx, b, p, bp);
+template <class P, int NF, bool DEP, bool TAB>
+void dispatch_nh(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
+                 const ParticleParams* tab) {
+  if (!(p.stages & ST_DIAG)) return launch_t<P, NF, DEP, 0, TAB>(ctx, b, p, bp, tab);
   const int nh = p.diag.nharm;
-  if (nh == 1) return launch_t<P, NF, DEP, 1>(ctx, b, p, bp);
-  if (nh == 2) return launch_t<P, NF, DEP, 2>(ctx, b, p, bp);
-  if (nh <= 4) return launch_t<P, NF, DEP, 4>(ctx, b, p, bp);
-  return launch_t<P, NF, DEP, 8>(ctx, b, p, bp);
+  if (nh == 1) return launch_t<P, NF, DEP, 1, TAB>(ctx, b, p, bp, tab);
+  if (nh == 2) return launch_t<P, NF, DEP, 2, TAB>(ctx, b, p, bp, tab);
+  if (nh <= 4) return launch_t<P, NF, DEP, 4, TAB>(ctx, b, p, bp, tab);
+  return launch_t<P, NF, DEP, 8, TAB>(ctx, b, p, bp, tab);
 }
 
 template <class P>
-void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp) {
+void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
+              const ParticleParams* tab) {
   const bool dep = (p.stages & ST_DEP) != 0;
   const bool single = !(p.stages & ST_PUSH) || p.push.nfield == 1;
-  if (single) {
-    if (dep) return dispatch_nh<P, 1, true>(ctx, b, p, bp);
-    return dispatch_nh<P, 1, false>(ctx, b, p, bp);
+  if (tab) {
+    require(single, "per-slice parameter tables serve single-field passes");
+    if (dep) return dispatch_nh<P, 1, true, true>(ctx, b, p, bp, tab);
+    return dispatch_nh<P, 1, false, true>(ctx, b, p, bp, tab);
   }
-  if (dep) return dispatch_nh<P, 0, true>(ctx, b, p, bp);
-  return dispatch_nh<P, 0, false>(ctx, b, p, bp);
+  if (single) {
+    if (dep) return dispatch_nh<P, 1, true, false>(ctx, b, p, bp, nullptr);
+    return dispatch_nh<P, 1, false, false>(ctx, b, p, bp, nullptr);
+  }
+  if (dep) return dispatch_nh<P, 0, true, false>(ctx, b, p, bp, nullptr);
+  return dispatch_nh<P, 0, false, false>(ctx, b, p, bp, nullptr);
 }
 
 }  // namespace
@@ -932,7 +999,8 @@ void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell) {
   });
 }
 
-void launch_particle(Context* ctx, Beam* b, const ParticleParams& p) {
+void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
+                     const ParticleParams* d_table) {
   if (b->nslice <= 0) return;
   require(p.dep.window_cap * sizeof(double2) <= 96 * 1024 || !(p.stages & ST_DEP),
           "deposition window exceeds the LDS budget");
@@ -941,11 +1009,11 @@ void launch_particle(Context* ctx, Beam* b, const ParticleParams& p) {
   if (b->prec == G4H_FP64) {
     BeamPtr<double> bp{b->xd.get(), b->yd.get(), b->pxd.get(), b->pyd.get(), b->gd.get(),
                        b->theta.get()};
-    dispatch<double>(ctx, b, p, bp);
+    dispatch<double>(ctx, b, p, bp, d_table);
   } else {
     BeamPtr<float> bp{b->xf.get(), b->yf.get(), b->pxf.get(), b->pyf.get(), b->gf.get(),
                       b->theta.get()};
-    dispatch<float>(ctx, b, p, bp);
+    dispatch<float>(ctx, b, p, bp, d_table);
   }
 }
 

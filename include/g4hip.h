@@ -28,6 +28,8 @@ typedef struct g4h_context g4h_context;
 typedef struct g4h_beam g4h_beam;
 typedef struct g4h_field g4h_field;
 typedef struct g4h_run g4h_run;
+typedef struct g4h_batch g4h_batch;
+typedef struct g4h_server g4h_server;
 
 /* ------------------------------------------------------------------------ */
 /* Library / context (replaces genesis4_cuda::initialize/check/stream/        */
@@ -202,6 +204,84 @@ g4h_status g4h_run_advance(g4h_run* run, int nsteps, int* done);
 g4h_status g4h_run_records(const g4h_run* run, int* nrecords);
 /* Blocking copy of records [0, nrecords) (layouts as g4h_diag_*_raw). */
 g4h_status g4h_run_download_diag(g4h_run* run, double* beam_raw, double* field_raw);
+
+/* ---- batches of independent steady-state runs -------------------------
+ * No counterpart class in the reference (which runs one steady-state
+ * simulation per process, Gencore.cpp:161-254); serves taper scans and
+ * request streams of small runs.  Runs sharing (ngrid, gridmax, gamma0) are
+ * advanced together: one particle launch, one solver step per group and
+ * step, each run with its own step parameters (aw taper).  Every run is one
+ * slice (steady state: no slippage). */
+typedef struct g4h_batch_run {
+  int32_t npart, nbins, ngrid, shotnoise;
+  uint64_t seed;
+  double gridmax, current, gamma0, espread_rel;
+  double emitn_x, emitn_y, betax, betay, alphax, alphay;
+  double seed_power, seed_w0; /* Gaussian seed at its waist [W], [m]        */
+  const double* aw;           /* aw per step [nsteps] (taper), or NULL      */
+} g4h_batch_run;
+
+/* steps[nsteps] is the shared lattice (its aw is replaced per run when the
+ * run gives a taper); records are taken before the first step and after
+ * every step with outstep != 0, as for g4h_run. */
+g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solver_kind solver,
+                            int nruns, const g4h_batch_run* runs, const g4h_step* steps,
+                            int nsteps, double lambda0, int nharm, g4h_batch** out);
+void g4h_batch_destroy(g4h_batch* batch);
+g4h_status g4h_batch_advance(g4h_batch* batch, int nsteps, int* done);
+g4h_status g4h_batch_records(const g4h_batch* batch, int* nrecords);
+/* Records of run `run`: beam_raw[nrecords * G4H_BEAM_RAW(nharm)],
+ * field_raw[nrecords * G4H_FIELD_RAW] (layouts of g4h_diag_*_raw). */
+g4h_status g4h_batch_download(g4h_batch* batch, int run, double* beam_raw, double* field_raw);
+
+/* ---- request loop ------------------------------------------------------
+ * No counterpart in the reference (one simulation per process).  A server
+ * keeps several runs resident on one GPU: requests are admitted while their
+ * estimated HBM footprint fits `hbm_budget` and one of `nstreams` stream
+ * lanes is free; every admitted request (or batch of steady-state requests
+ * sharing lattice, precision and solver) runs asynchronously on its lane,
+ * so work of different requests overlaps on the device; finished requests
+ * free their memory for the next ones.  Latency runs from submission until
+ * the request's records are on the host. */
+typedef struct g4h_request {
+  int32_t kind; /* 0 steady state (one slice, batched), 1 time-dependent */
+  g4h_precision precision;
+  g4h_solver_kind solver;
+  int32_t nslice, npart, nbins, ngrid, shotnoise, nharm, periodic, nsteps;
+  uint64_t seed;
+  double gridmax, current, gamma0, espread_rel, chirp_rel, emitn, beta;
+  double seed_power, seed_w0; /* Gaussian seed at its waist               */
+  double lambda0, sample;
+  const g4h_step* steps; /* [nsteps], copied at submission               */
+  const double* aw;      /* per-step aw (taper) or NULL, copied          */
+} g4h_request;
+
+typedef struct g4h_result_info {
+  int32_t status, nrecords;
+  double latency_s; /* submission -> records on the host               */
+  double queued_s;  /* submission -> admission                         */
+} g4h_result_info;
+
+typedef struct g4h_server_stats_t {
+  int64_t completed;
+  double p50_s, p99_s, mean_s;       /* latency over completed requests  */
+  double particle_slice_steps_per_s; /* completed work / (last end - first submission) */
+  uint64_t hbm_in_use;
+} g4h_server_stats_t;
+
+g4h_status g4h_server_create(int device, uint64_t hbm_budget, int nstreams, g4h_server** out);
+/* Waits for the queue to drain, then releases everything. */
+void g4h_server_destroy(g4h_server* server);
+g4h_status g4h_server_submit(g4h_server* server, const g4h_request* request, int64_t* id);
+/* Blocks until request `id` finished (timeout_s < 0: no limit); returns
+ * the request's own status. */
+g4h_status g4h_server_wait(g4h_server* server, int64_t id, double timeout_s,
+                           g4h_result_info* info);
+/* Records of a finished request: beam_raw[nrecords * nslice * G4H_BEAM_RAW(nharm)],
+ * field_raw[nrecords * nslice * G4H_FIELD_RAW]. */
+g4h_status g4h_server_result(g4h_server* server, int64_t id, double* beam_raw, double* field_raw);
+g4h_status g4h_server_release(g4h_server* server, int64_t id);
+g4h_status g4h_server_stats(g4h_server* server, g4h_server_stats_t* stats);
 
 /* Per-kernel timing with HIP events on the context's stream.  While on,
  * every launch is bracketed by events; g4h_profile_read returns for kernel

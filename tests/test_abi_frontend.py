@@ -148,3 +148,12 @@ def test_seed_gaussian_normalisation():
     p = np.sum(re[:51 * 51] ** 2 + im[:51 * 51] ** 2) * (dg * 510999.06 / ks) ** 2 / 376.73
     assert p == pytest.approx(1.0e3, rel=1e-12)
     assert np.all(im == 0) and np.array_equal(re[:2601], re[2601:5202])
+
+
+def test_server_and_batch_need_a_device(g4lib):
+    if api.device_count() != 0:
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    assert g4lib.g4h_server_create(0, 1 << 30, 2, C.byref(h)) == 6  # G4H_ERR_NO_DEVICE
+    assert g4lib.g4h_server_create(0, 1 << 30, 0, C.byref(h)) == 1  # bad lane count
+    assert g4lib.g4h_batch_create(None, 0, 0, 1, None, None, 1, 1e-9, 1, C.byref(h)) == 1
