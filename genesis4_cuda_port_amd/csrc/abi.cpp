@@ -632,9 +632,12 @@ g4h_status g4h_field_init_solver(g4h_field* f, g4h_solver_kind kind, int filter,
   f->sig = sig;
   // FieldSolverFFTCUDA::initSourceFilter (FieldSolverFFTCUDA.cpp:213-221)
   f->filter = kind == G4H_SOLVER_FFT && filter && !(sig <= 0 || xc <= 0 || yc <= 0);
+  G4_HIP(hipStreamSynchronize(f->ctx->stream));  // tables may still be read
   f->fft_tw.release();  // tables are rebuilt on the next init
   f->fft_sigmoid.release();
-  f->adi_c.release();
+  f->delz_tables.clear();
+  f->adi_c = f->adi_cbet = f->adi_cwet = nullptr;
+  f->fft_prop = nullptr;
   f->adi_delz = 0.0;
   f->fft_delz = 0.0;
   f->solver_ready = false;

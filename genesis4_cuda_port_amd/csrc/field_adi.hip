@@ -307,13 +307,13 @@ void adi_step(Field* f, bool source_on, bool stats) {
   const int add = source_on ? 1 : 0;
   f->ctx->launch(G4H_K_ADI_X, [&] {
     hipLaunchKernelGGL(adi_x_kernel<P>, dim3(f->nslice), dim3(nt), ldsx, f->ctx->stream, u,
-                       f->adi_y.get(), f->nslice, G, cs, f->adi_c.get(), f->adi_cbet.get(),
-                       f->adi_cwet.get(), src, f->src_rect.get(), f->first, add);
+                       f->adi_y.get(), f->nslice, G, cs, f->adi_c, f->adi_cbet,
+                       f->adi_cwet, src, f->src_rect.get(), f->first, add);
   });
   f->ctx->launch(G4H_K_ADI_Y, [&] {
     hipLaunchKernelGGL(adi_y_kernel<P>, dim3(f->nslice), dim3(nt), ldsy, f->ctx->stream, u,
-                       f->adi_y.get(), f->nslice, G, cs, f->adi_c.get(), f->adi_cbet.get(),
-                       f->adi_cwet.get(), src, f->src_rect.get(), f->first, add,
+                       f->adi_y.get(), f->nslice, G, cs, f->adi_c, f->adi_cbet,
+                       f->adi_cwet, src, f->src_rect.get(), f->first, add,
                        stats ? f->plane_stats.get() : nullptr);
   });
 }
@@ -323,50 +323,57 @@ void adi_step(Field* f, bool source_on, bool stats) {
 // Thomas factors of FieldSolverADICUDA::init (FieldSolverADICUDA.cpp:821-866).
 void adi_prepare(Field* f, double delz) {
   const int G = f->ngrid;
-  if (delz == f->adi_delz && G == f->adi_ngrid && f->adi_c.size() == static_cast<size_t>(G)) return;
-  const double rtmp = 0.25 * delz / (f->xks * f->dgrid * f->dgrid);
-  using cd = std::complex<double>;
-  std::vector<double> mupp(G), mmid(G), mlow(G);
-  mupp[0] = rtmp;
-  mmid[0] = -2 * rtmp;
-  mlow[0] = 0;
-  for (int i = 1; i < G - 1; i++) {
-    mupp[i] = rtmp;
-    mmid[i] = -2 * rtmp;
-    mlow[i] = rtmp;
+  if (f->adi_c && delz == f->adi_delz) return;
+  const double key = delz;
+  auto it = f->delz_tables.find(key);
+  if (it == f->delz_tables.end()) {
+    if (f->delz_tables.size() >= 64) {  // many distinct step lengths: start over
+      G4_HIP(hipStreamSynchronize(f->ctx->stream));
+      f->delz_tables.clear();
+    }
+    const double rtmp = 0.25 * delz / (f->xks * f->dgrid * f->dgrid);
+    using cd = std::complex<double>;
+    std::vector<double> mupp(G), mmid(G), mlow(G);
+    mupp[0] = rtmp;
+    mmid[0] = -2 * rtmp;
+    mlow[0] = 0;
+    for (int i = 1; i < G - 1; i++) {
+      mupp[i] = rtmp;
+      mmid[i] = -2 * rtmp;
+      mlow[i] = rtmp;
+    }
+    mupp[G - 1] = 0;
+    mmid[G - 1] = -2 * rtmp;
+    mlow[G - 1] = rtmp;
+    std::vector<cd> w1(G), w2(G), c(G), cbet(G), cwet(G);
+    for (int i = 0; i < G; i++) {
+      w1[i] = cd(0.0, -mupp[i]);
+      w2[i] = cd(1.0, -mmid[i]);
+      c[i] = cd(0.0, -mlow[i]);
+    }
+    cbet[0] = cd(1.0, 0.0) / w2[0];
+    cwet[0] = cd(0.0, 0.0);
+    for (int i = 1; i < G; i++) {
+      cwet[i] = w1[i - 1] * cbet[i - 1];
+      cbet[i] = cd(1.0, 0.0) / (w2[i] - c[i] * cwet[i]);
+    }
+    Field::DelzTable t;
+    t.host.resize(3 * static_cast<size_t>(G));
+    for (int i = 0; i < G; ++i) {
+      t.host[i] = make_double2(c[i].real(), c[i].imag());
+      t.host[G + i] = make_double2(cbet[i].real(), cbet[i].imag());
+      t.host[2 * G + i] = make_double2(cwet[i].real(), cwet[i].imag());
+    }
+    t.dev.alloc(f->ctx, 3 * static_cast<size_t>(G));
+    G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * 3 * G,
+                          hipMemcpyHostToDevice, f->ctx->stream));
+    it = f->delz_tables.emplace(key, std::move(t)).first;
   }
-  mupp[G - 1] = 0;
-  mmid[G - 1] = -2 * rtmp;
-  mlow[G - 1] = rtmp;
-  std::vector<cd> w1(G), w2(G), c(G), cbet(G), cwet(G);
-  for (int i = 0; i < G; i++) {
-    w1[i] = cd(0.0, -mupp[i]);
-    w2[i] = cd(1.0, -mmid[i]);
-    c[i] = cd(0.0, -mlow[i]);
-  }
-  cbet[0] = cd(1.0, 0.0) / w2[0];
-  cwet[0] = cd(0.0, 0.0);
-  for (int i = 1; i < G; i++) {
-    cwet[i] = w1[i - 1] * cbet[i - 1];
-    cbet[i] = cd(1.0, 0.0) / (w2[i] - c[i] * cwet[i]);
-  }
-  std::vector<double2> hc(G), hb(G), hw(G);
-  for (int i = 0; i < G; ++i) {
-    hc[i] = make_double2(c[i].real(), c[i].imag());
-    hb[i] = make_double2(cbet[i].real(), cbet[i].imag());
-    hw[i] = make_double2(cwet[i].real(), cwet[i].imag());
-  }
-  if (f->adi_c.size() != static_cast<size_t>(G)) {
-    f->adi_c.alloc(f->ctx, G);
-    f->adi_cbet.alloc(f->ctx, G);
-    f->adi_cwet.alloc(f->ctx, G);
-  }
-  G4_HIP(hipMemcpyAsync(f->adi_c.get(), hc.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
-  G4_HIP(hipMemcpyAsync(f->adi_cbet.get(), hb.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
-  G4_HIP(hipMemcpyAsync(f->adi_cwet.get(), hw.data(), sizeof(double2) * G, hipMemcpyHostToDevice, f->ctx->stream));
-  G4_HIP(hipStreamSynchronize(f->ctx->stream));
+  const double2* d = it->second.dev.get();
+  f->adi_c = d;
+  f->adi_cbet = d + G;
+  f->adi_cwet = d + 2 * G;
   f->adi_delz = delz;
-  f->adi_ngrid = G;
 }
 
 void adi_solve(Field* f, bool source_on, bool stats) {

@@ -564,7 +564,7 @@ void rows_reg(Field* f, typename C2<P>::T* data, int mode, double scale) {
   const int64_t nblk = std::min<int64_t>((ngroups + C::kWaves - 1) / C::kWaves, 8192);
   hipLaunchKernelGGL((fft_rows_reg_kernel<P, G>), dim3(static_cast<unsigned>(nblk)),
                      dim3(C::kThreads), 0, f->ctx->stream, data, nrows, f->fft_tw.get(),
-                     f->fft_prop.get(), mode, scale);
+                     f->fft_prop, mode, scale);
 }
 
 template <class P, int G>
@@ -574,7 +574,7 @@ void cols_reg(Field* f, typename C2<P>::T* data, int mode, double scale, int add
   using C = RegCfg<P, G>;
   constexpr int ntile = G / C::kTile;
   hipLaunchKernelGGL((fft_cols_reg_kernel<P, G>), dim3(f->nslice * ntile), dim3(C::kThreads), 0,
-                     f->ctx->stream, data, f->nslice, f->fft_tw.get(), f->fft_prop.get(),
+                     f->ctx->stream, data, f->nslice, f->fft_tw.get(), f->fft_prop,
                      f->fft_sigmoid.get(), mode, scale, static_cast<const T2*>(f->src()),
                      f->src_rect.get(), f->first, add_src, f->fft_work.get(), partials);
 }
@@ -623,7 +623,7 @@ void run_rows(Field* f, typename C2<P>::T* data, int mode, double scale) {
     const size_t lds = sizeof(T2) * (G + static_cast<size_t>(c.row_lines) * (G + 1));
     hipLaunchKernelGGL(fft_rows_dft_kernel<P>, dim3(static_cast<unsigned>(nblk)), dim3(256), lds,
                        f->ctx->stream, data, nrows, G, c.row_lines, f->fft_tw.get(),
-                       f->fft_prop.get(), mode, scale);
+                       f->fft_prop, mode, scale);
   });
 }
 
@@ -644,7 +644,7 @@ void run_cols(Field* f, typename C2<P>::T* data, int mode, double scale, int add
     const size_t lds = sizeof(T2) * (G + static_cast<size_t>(G) * (c.tc + 1));
     hipLaunchKernelGGL(fft_cols_dft_kernel<P>, dim3(f->nslice * ntile), dim3(c.col_threads), lds,
                        f->ctx->stream, data, f->nslice, G, c.tc, f->fft_tw.get(),
-                       f->fft_prop.get(), f->fft_sigmoid.get(), mode, scale,
+                       f->fft_prop, f->fft_sigmoid.get(), mode, scale,
                        static_cast<const T2*>(f->src()), f->src_rect.get(), f->first, add_src,
                        f->fft_work.get(), partials);
   });
@@ -709,7 +709,7 @@ void fft_prepare(Field* f, double delz) {
     f->fft_tw.alloc(f->ctx, G);
     G4_HIP(hipMemcpy(f->fft_tw.get(), tw.data(), sizeof(double2) * G, hipMemcpyHostToDevice));
     f->fft_delz = 0.0;
-    f->fft_prop.alloc(f->ctx, G);
+    f->fft_prop = nullptr;
     if (f->filter) {
       // sigmoid filter at shifted index, FieldSolverFFTCUDA.cpp:297-316
       const double shift = -0.5 * static_cast<double>(G - 1);
@@ -730,25 +730,35 @@ void fft_prepare(Field* f, double delz) {
                        hipMemcpyHostToDevice));
     }
   }
-  if (delz != f->fft_delz) {
-    // k2 at frequency index m: dx(m) = ((m - (G+1)/2) mod G) - (G-1)/2 and
-    // k2 = -(dx^2 + dy^2) dk^2/(2 xks) (FieldSolverFFTCUDA.cpp:297-309);
-    // the x (or y) factor is exp(-i dx^2 dk^2 delz / (2 xks)).
-    // Power-of-two sizes use the four-step register transforms, whose
-    // spectrum sits at permuted positions: entry e holds the factor of
-    // frequency spectrum_index(e).
-    const double dk = 4. * std::asin(1.) / (static_cast<double>(G) * f->dgrid);
-    std::vector<double2> pr(G);
-    for (int e = 0; e < G; ++e) {
-      const int m = is_pow2(G) ? fftr::spectrum_index(e, G) : e;
-      const int ix = ((m - (G + 1) / 2) % G + G) % G;
-      const double dx = static_cast<double>(ix) - 0.5 * static_cast<double>(G - 1);
-      const double ph = -dx * dx * dk * dk / 2. / f->xks * delz;
-      pr[e] = make_double2(std::cos(ph), std::sin(ph));
+  if (!f->fft_prop || delz != f->fft_delz) {
+    auto it = f->delz_tables.find(delz);
+    if (it == f->delz_tables.end()) {
+      if (f->delz_tables.size() >= 64) {  // many distinct step lengths: start over
+        G4_HIP(hipStreamSynchronize(f->ctx->stream));
+        f->delz_tables.clear();
+      }
+      // k2 at frequency index m: dx(m) = ((m - (G+1)/2) mod G) - (G-1)/2 and
+      // k2 = -(dx^2 + dy^2) dk^2/(2 xks) (FieldSolverFFTCUDA.cpp:297-309);
+      // the x (or y) factor is exp(-i dx^2 dk^2 delz / (2 xks)).
+      // Power-of-two sizes use the four-step register transforms, whose
+      // spectrum sits at permuted positions: entry e holds the factor of
+      // frequency spectrum_index(e).
+      const double dk = 4. * std::asin(1.) / (static_cast<double>(G) * f->dgrid);
+      Field::DelzTable t;
+      t.host.resize(G);
+      for (int e = 0; e < G; ++e) {
+        const int m = is_pow2(G) ? fftr::spectrum_index(e, G) : e;
+        const int ix = ((m - (G + 1) / 2) % G + G) % G;
+        const double dx = static_cast<double>(ix) - 0.5 * static_cast<double>(G - 1);
+        const double ph = -dx * dx * dk * dk / 2. / f->xks * delz;
+        t.host[e] = make_double2(std::cos(ph), std::sin(ph));
+      }
+      t.dev.alloc(f->ctx, G);
+      G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * G,
+                            hipMemcpyHostToDevice, f->ctx->stream));
+      it = f->delz_tables.emplace(delz, std::move(t)).first;
     }
-    G4_HIP(hipMemcpyAsync(f->fft_prop.get(), pr.data(), sizeof(double2) * G, hipMemcpyHostToDevice,
-                          f->ctx->stream));
-    G4_HIP(hipStreamSynchronize(f->ctx->stream));
+    f->fft_prop = it->second.dev.get();
     f->fft_delz = delz;
   }
 }

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -186,15 +187,24 @@ struct Field {
   bool filter = false;
   double xc = 1, yc = 1, sig = 1;
   bool solver_ready = false;
+  // Step-length dependent solver tables, one per distinct delz of the
+  // lattice: built once, uploaded asynchronously (the host copy lives with
+  // the entry), selected per step without a host synchronisation.
+  struct DelzTable {
+    std::vector<double2> host;  // [3G] ADI c, cbet, cwet, or [G] FFT factor
+    DevBuf<double2> dev;
+  };
+  std::map<double, DelzTable> delz_tables;
   // ADI
   double adi_delz = 0.0;
-  int adi_ngrid = 0;
-  DevBuf<double2> adi_c, adi_cbet, adi_cwet;  // [ngrid]
-  DevBuf<double2> adi_y;                      // FP64 intermediates [nslice*ngrid^2]
+  const double2* adi_c = nullptr;  // current tables [ngrid] (views into delz_tables)
+  const double2* adi_cbet = nullptr;
+  const double2* adi_cwet = nullptr;
+  DevBuf<double2> adi_y;  // FP64 intermediates [nslice*ngrid^2]
   // FFT
   double fft_delz = 0.0;
-  DevBuf<double2> fft_tw;    // exp(-2 pi i k / G), [G]
-  DevBuf<double2> fft_prop;  // separable propagator factor [G]
+  DevBuf<double2> fft_tw;            // four-step twiddles / DFT twiddles, [G]
+  const double2* fft_prop = nullptr;  // current separable propagator factor [G]
   DevBuf<double> fft_sigmoid;  // [G*G] (filter only)
   DevBuf<double2> fft_work;    // filtered-source work planes (filter only)
   // diagnostics
