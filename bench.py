@@ -161,6 +161,44 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int):
                       f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
 
 
+def taper_scan(prec: int, nruns: int, seed: int):
+    """Batched taper scan: nruns independent steady-state runs (grid drawn
+    from 101/151/201/255, 4096/8192/16384 particles, aw(z) = aw0 (1 - a z -
+    b z^2) per module with a, b drawn from [0, 2e-3]) over 10 modules of 100
+    periods with FODO focusing, ADI, MIXED or FP64, records every 10 steps.
+    Returns throughput in particle-slice-steps/s of the advance (state and
+    tables already on the device)."""
+    from genesis4_cuda_port_amd import api
+    rng = np.random.default_rng(seed)
+    nmod = 10
+    steps = api.lattice_fodo(LAMBDAU, [AW] * nmod, 100, 46, 3, 1.5, 10)
+    ns = len(steps)
+    # module index of each step (100 one-period steps per module)
+    z = np.floor(np.arange(ns) * nmod / ns) / nmod
+    runs = []
+    for _ in range(nruns):
+        a, b = rng.uniform(0.0, 2e-3, 2)
+        runs.append(dict(npart=int(rng.choice([4096, 8192, 16384])), nbins=4,
+                         ngrid=int(rng.choice([101, 151, 201, 255])), gridmax=0.3e-3,
+                         current=3000.0, gamma0=GAMMA0, espread_rel=1e-4, emitn=0.4e-6,
+                         beta=15.0, shotnoise=False, seed=int(rng.integers(1, 1 << 31)),
+                         seed_power=1e3, seed_w0=50e-6, aw=AW * (1 - a * z - b * z * z)))
+    with api.Context(0) as ctx:
+        batch = api.Batch(ctx, prec, api.ADI, runs, steps, LAMBDA0)
+        batch.advance(1)
+        ctx.synchronize()
+        t = time.perf_counter()
+        batch.advance(ns - 1)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        batch.close()
+    parts = sum(r["npart"] for r in runs)
+    grids = sorted({r["ngrid"] for r in runs})
+    return {"value": parts * (ns - 1) / dt, "unit": "particle-slice-steps/s",
+            "ms_per_step": dt / (ns - 1) * 1e3, "runs": nruns, "steps": ns - 1,
+            "particles": parts, "grids": grids}
+
+
 def serving_requests(n: int, seed: int):
     """The request stream: 70 % steady-state runs (grid 101/151/201/255,
     4096/8192/16384 particles, linear + quadratic aw taper with coefficients
@@ -229,6 +267,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-runs", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1536)
     ap.add_argument("--cpu-steps", type=int, default=12)
@@ -315,6 +355,14 @@ def main() -> int:
         log("cpu baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
                                            a.cpu_steps)
+    if not a.no_batch:
+        scan = {"workload": f"taper scan of {a.batch_runs} steady-state runs in one batch "
+                            "(uneven grid and particle count, 10 x 100-period modules, ADI)",
+                "seed": 1234573}
+        for name, prec in (("mixed", api.MIXED), ("fp64", api.FP64)):
+            log(f"taper scan ({name})")
+            scan[name] = taper_scan(prec, a.batch_runs, 1234573)
+        out["taper_scan"] = scan
     out["latency_p50_p99"] = None
     if not a.no_serve:
         # capacity with every request queued at once, then Poisson arrivals at

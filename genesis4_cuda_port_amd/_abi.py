@@ -23,6 +23,7 @@ STATUS_NAMES = {
 }
 FP64, MIXED = 0, 1
 ADI, FFT = 0, 1
+TUNE_ADI, TUNE_SPLIT = 0, 1  # kernel families of g4h_context_set_variant
 
 BEAM_STATS = 23
 FIELD_RAW = 9
@@ -129,6 +130,9 @@ PROTOTYPES = {
     "g4h_context_destroy": (None, [_vp]),
     "g4h_context_synchronize": (C.c_int, [_vp]),
     "g4h_context_memory": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "g4h_context_set_variant": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "g4h_context_get_variant": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                          C.POINTER(C.c_int)]),
     "g4h_beam_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp, C.c_double, C.POINTER(_vp)]),
     "g4h_beam_destroy": (None, [_vp]),
     "g4h_beam_upload": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),

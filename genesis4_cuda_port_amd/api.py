@@ -87,6 +87,17 @@ class Context:
         check(lib().g4h_context_memory(self.h, C.byref(v)))
         return int(v.value)
 
+    def set_variant(self, family: int, variant: int):
+        """Pin a kernel family to one variant (-1: autotune)."""
+        check(lib().g4h_context_set_variant(self.h, family, variant))
+
+    def get_variant(self, family: int, shape) -> int:
+        """Variant the tuner decided for `shape` (-1: still measuring)."""
+        arr = (C.c_int * len(shape))(*shape)
+        v = C.c_int()
+        check(lib().g4h_context_get_variant(self.h, family, arr, len(shape), C.byref(v)))
+        return int(v.value)
+
     def profile(self, on: bool):
         check(lib().g4h_profile_enable(self.h, 1 if on else 0))
 

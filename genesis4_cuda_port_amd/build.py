@@ -31,6 +31,7 @@ SOURCES = [
     "run.cpp",
     "batch.cpp",
     "serve.cpp",
+    "tune.cpp",
     "frontend.cpp",
     "deck.cpp",
     "deck_run.cpp",

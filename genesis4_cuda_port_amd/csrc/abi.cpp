@@ -241,23 +241,6 @@ int slippage_host(Field* f, double slippage, double sample, int timerun, int per
 
 using namespace g4;
 
-#define G4_API_BEGIN try {
-#define G4_API_END                                                  \
-  return G4H_OK;                                                    \
-  }                                                                 \
-  catch (const g4::Error& e) {                                      \
-    g4::set_last_error(e.what());                                   \
-    return e.code;                                                  \
-  }                                                                 \
-  catch (const std::bad_alloc&) {                                   \
-    g4::set_last_error("host allocation failed");                   \
-    return G4H_ERR_OUT_OF_MEMORY;                                   \
-  }                                                                 \
-  catch (const std::exception& e) {                                 \
-    g4::set_last_error(e.what());                                   \
-    return G4H_ERR_DEVICE;                                          \
-  }
-
 static void release_context(g4h_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
@@ -266,6 +249,7 @@ static void release_context(g4h_context* ctx) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  ctx->tune_release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -13,6 +13,10 @@
 // host from the same make_trans / make_push / make_dep as every other path
 // and copied through a pinned ring of tables.
 //
+// Each group has its own context (its own stream), so the groups' launches
+// overlap on the device; g4h_batch_advance joins them into the caller's
+// stream before it returns.
+//
 // Steady state only: one slice per run, no slippage (Gencore.cpp:229 is a
 // no-op there, ControlCUDA.cpp:150).
 #include <algorithm>
@@ -40,6 +44,7 @@ void ok(g4h_status s) {
 }
 
 struct Group {
+  g4h_context* gctx = nullptr;  // own stream: groups run concurrently
   int G = 0;
   double gridmax = 0, gamma0 = 0;
   std::vector<int> runs;
@@ -48,6 +53,7 @@ struct Group {
   DevBuf<ParticleParams> dtab;       // kRing tables of runs.size() entries
   ParticleParams* htab = nullptr;    // pinned host copy of the ring
   hipEvent_t used[kRing] = {};       // last launch that read each ring slot
+  hipEvent_t tail = nullptr;         // end of the group's work so far
   bool pending[kRing] = {};
   DevBuf<double> beam_hist, field_hist;
 };
@@ -73,7 +79,12 @@ struct g4h_batch {
       if (g.beam) g4h_beam_destroy(g.beam);
       for (hipEvent_t e : g.used)
         if (e) (void)hipEventDestroy(e);
+      if (g.tail) (void)hipEventDestroy(g.tail);
       if (g.htab) (void)hipHostFree(g.htab);
+      g.dtab.release();
+      g.beam_hist.release();
+      g.field_hist.release();
+      if (g.gctx) g4h_context_destroy(g.gctx);
     }
   }
 
@@ -89,7 +100,7 @@ struct g4h_batch {
       p.stages = ST_DIAG;
       p.diag.raw = g.beam_hist.get();
       p.diag.nharm = nharm;
-      launch_particle(ctx, g.beam, p);
+      launch_particle(g.gctx, g.beam, p);
       field_plane_stats(g.field);
       field_map_stats(g.field, g.field_hist.get());
     }
@@ -134,9 +145,9 @@ struct g4h_batch {
       }
       ParticleParams* dtab = g.dtab.get() + static_cast<size_t>(sl) * n;
       G4_HIP(hipMemcpyAsync(dtab, tab, sizeof(ParticleParams) * n, hipMemcpyHostToDevice,
-                            ctx->stream));
-      launch_particle(ctx, g.beam, tab[0], dtab);
-      G4_HIP(hipEventRecord(g.used[sl], ctx->stream));
+                            g.gctx->stream));
+      launch_particle(g.gctx, g.beam, tab[0], dtab);
+      G4_HIP(hipEventRecord(g.used[sl], g.gctx->stream));
       g.pending[sl] = true;
       field_solve(g.field, src_on, out);
       if (out) field_map_stats(g.field, g.field_hist.get() + static_cast<size_t>(nrec) * field_rec(g));
@@ -144,19 +155,6 @@ struct g4h_batch {
     if (out) ++nrec;
   }
 };
-
-#define G4_API_BEGIN try {
-#define G4_API_END                     \
-  return G4H_OK;                       \
-  }                                    \
-  catch (const g4::Error& e) {         \
-    g4::set_last_error(e.what());      \
-    return e.code;                     \
-  }                                    \
-  catch (const std::exception& e) {    \
-    g4::set_last_error(e.what());      \
-    return G4H_ERR_DEVICE;             \
-  }
 
 extern "C" {
 
@@ -246,18 +244,20 @@ g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solve
                              im.data() + po));
       }
     }
-    ok(g4h_beam_create(ctx, precision, n, off.data(), cur.data(), g.gamma0, &g.beam));
+    ok(g4h_context_create(ctx->device, &g.gctx));
+    ok(g4h_beam_create(g.gctx, precision, n, off.data(), cur.data(), g.gamma0, &g.beam));
     ok(g4h_beam_upload(g.beam, x.data(), y.data(), px.data(), py.data(), ga.data(), th.data()));
-    ok(g4h_field_create(ctx, precision, n, g.G, g.gridmax, lambda0, lambda0, 1, &g.field));
+    ok(g4h_field_create(g.gctx, precision, n, g.G, g.gridmax, lambda0, lambda0, 1, &g.field));
     ok(g4h_field_upload(g.field, re.data(), im.data(), 0));
     ok(g4h_field_init_solver(g.field, solver, 0, 1.0, 1.0, 1.0, 0));
-    g.dtab.alloc(ctx, static_cast<size_t>(kRing) * n);
+    g.dtab.alloc(g.gctx, static_cast<size_t>(kRing) * n);
     void* h = nullptr;
     G4_HIP(hipHostMalloc(&h, sizeof(ParticleParams) * kRing * n, hipHostMallocDefault));
     g.htab = static_cast<ParticleParams*>(h);
     for (hipEvent_t& e : g.used) G4_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    g.beam_hist.alloc(ctx, static_cast<size_t>(b->cap) * b->beam_rec(g));
-    g.field_hist.alloc(ctx, static_cast<size_t>(b->cap) * b->field_rec(g));
+    G4_HIP(hipEventCreateWithFlags(&g.tail, hipEventDisableTiming));
+    g.beam_hist.alloc(g.gctx, static_cast<size_t>(b->cap) * b->beam_rec(g));
+    g.field_hist.alloc(g.gctx, static_cast<size_t>(b->cap) * b->field_rec(g));
   }
   b->record_initial();
   *out = b.release();
@@ -269,7 +269,8 @@ void g4h_batch_destroy(g4h_batch* b) {
   if (!b) return;
   Context* ctx = b->ctx;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  for (const Group& g : b->groups)
+    if (g.gctx) (void)hipStreamSynchronize(g.gctx->stream);
   delete b;
   context_child_removed(ctx);
 }
@@ -281,6 +282,12 @@ g4h_status g4h_batch_advance(g4h_batch* b, int nsteps, int* done) {
   b->ctx->activate();
   const int end = std::min<int>(static_cast<int>(b->steps.size()), b->done + nsteps);
   for (; b->done < end; ++b->done) b->step(b->done);
+  // the context's stream waits for every group, so work enqueued on it later
+  // (or a synchronisation of it) comes after this advance
+  for (Group& g : b->groups) {
+    G4_HIP(hipEventRecord(g.tail, g.gctx->stream));
+    G4_HIP(hipStreamWaitEvent(b->ctx->stream, g.tail, 0));
+  }
   if (done) *done = b->done;
   G4_API_END
 }
@@ -297,9 +304,9 @@ g4h_status g4h_batch_download(g4h_batch* b, int run, double* beam_raw, double* f
   require(b, "batch is null");
   require(run >= 0 && run < static_cast<int>(b->where.size()), "run out of range");
   b->ctx->activate();
-  G4_HIP(hipStreamSynchronize(b->ctx->stream));
   const auto [gi, si] = b->where[static_cast<size_t>(run)];
   const Group& g = b->groups[static_cast<size_t>(gi)];
+  G4_HIP(hipStreamSynchronize(g.gctx->stream));
   const size_t br = G4H_BEAM_RAW(b->nharm);
   for (int r = 0; r < b->nrec; ++r) {
     if (beam_raw)

@@ -41,8 +41,12 @@ struct A2<float> {
   using T = float2;
 };
 
+// Floating-point contraction is off in this file: the fused multiply-adds
+// below are the only ones, so every kernel variant rounds identically.
+#pragma clang fp contract(off)
+
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -50,7 +54,7 @@ __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_doub
 // rhs = center + i*cs*((a + b) - 2 center) [+ s]
 __device__ __forceinline__ double2 rhs_of(double2 center, double2 a, double2 b, double cs) {
   const double2 lap = zsub(zadd(a, b), make_double2(center.x * 2.0, center.y * 2.0));
-  return make_double2(center.x + -cs * lap.y, center.y + cs * lap.x);
+  return make_double2(fma(-cs, lap.y, center.x), fma(cs, lap.x, center.y));
 }
 
 template <class T2>
@@ -292,11 +296,296 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
   }
 }
 
+// ---- lane variant ---------------------------------------------------------
+// For few planes (steady-state batches, short time windows) the workgroup
+// per plane above leaves most of the GPU idle and pays a barrier per row.
+// Here one 64-lane wave owns 64 rows (x sweep) or 64 columns (y sweep) of a
+// plane, so a plane of G = 255 is four independent waves.  The Laplacian
+// neighbours across lanes come from DPP wave shifts; the two edge lanes of
+// the wave load the row / column beyond the block themselves.  Every
+// element goes through the same operations in the same order as in the
+// workgroup kernels, so both variants give bit-identical fields and, with
+// the per-block partial sums reduced in block order, bit-identical stats.
+
+constexpr int kLane = 64;
+constexpr int kCh = 8;  // elements per lane per chunk
+
+// lane i <- lane i-1 (wave_shr:1); lane 0 keeps `edge`
+__device__ __forceinline__ double2 from_lane_below(double2 v, double2 edge) {
+  int r[4];
+  const int* a = reinterpret_cast<const int*>(&v);
+  const int* e = reinterpret_cast<const int*>(&edge);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_update_dpp(e[k], a[k], 0x138, 0xf, 0xf, false);
+  double2 o;
+  __builtin_memcpy(&o, r, sizeof(o));
+  return o;
+}
+// lane i <- lane i+1 (wave_shl:1); lane 63 keeps `edge`
+__device__ __forceinline__ double2 from_lane_above(double2 v, double2 edge) {
+  int r[4];
+  const int* a = reinterpret_cast<const int*>(&v);
+  const int* e = reinterpret_cast<const int*>(&edge);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_update_dpp(e[k], a[k], 0x130, 0xf, 0xf, false);
+  double2 o;
+  __builtin_memcpy(&o, r, sizeof(o));
+  return o;
+}
+
+// x sweep: lane t of block b owns row iy = 64 b + t and walks its columns.
+template <class P>
+__global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u, double2* ys,
+                                                           int nslice, int G, double cs,
+                                                           const double2* __restrict__ c_d,
+                                                           const double2* __restrict__ cbet_d,
+                                                           const double2* __restrict__ cwet_d,
+                                                           const typename A2<P>::T* src,
+                                                           const int4* rect, int first,
+                                                           int add_src) {
+  using T2 = typename A2<P>::T;
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
+  const int iy = blockIdx.y * kLane + t;
+  const bool act = iy < G;
+  // the row beyond the block, loaded by the wave's edge lanes
+  const int ey = t == 0 ? iy - 1 : (t == kLane - 1 ? iy + 1 : -1);
+  const bool eact = ey >= 0 && ey < G;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  T2* up = u + p * plane;
+  double2* yp = ys + p * plane;
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const bool srow = sp && act && iy >= rc.y && iy < rc.w;
+  const T2* row = up + static_cast<int64_t>(act ? iy : 0) * G;
+  const T2* erow = up + static_cast<int64_t>(eact ? ey : 0) * G;
+  double2* yrow = yp + static_cast<int64_t>(act ? iy : 0) * G;
+  const double2 zero = make_double2(0.0, 0.0);
+  T2 tz;
+  tz.x = 0;
+  tz.y = 0;
+  // chunk loads run one chunk ahead of the recurrence
+  T2 nc[kCh], ne[kCh], ns[kCh];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) {
+      const int k = k0 + c;
+      nc[c] = tz;
+      ne[c] = tz;
+      ns[c] = tz;
+      if (k < G) {
+        if (act) nc[c] = row[k];
+        if (eact) ne[c] = erow[k];
+        if (srow && k >= rc.x && k < rc.z) ns[c] = sp[static_cast<int64_t>(iy) * G + k];
+      }
+    }
+  };
+  load(0);
+  double2 yprev = zero;
+  for (int k0 = 0; k0 < G; k0 += kCh) {
+    double2 cen[kCh], edg[kCh], sv[kCh];
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) {
+      cen[c] = to_d(nc[c]);
+      edg[c] = to_d(ne[c]);
+      sv[c] = to_d(ns[c]);
+    }
+    if (k0 + kCh < G) load(k0 + kCh);
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) {
+      const int k = k0 + c;
+      if (k < G) {  // uniform
+        const double2 above = from_lane_below(cen[c], edg[c]);  // row iy - 1
+        const double2 below = from_lane_above(cen[c], edg[c]);  // row iy + 1
+        double2 r = rhs_of(cen[c], below, above, cs);
+        if (srow && k >= rc.x && k < rc.z) r = zadd(r, sv[c]);
+        yprev = zmul(zsub(r, zmul(c_d[k], yprev)), cbet_d[k]);
+        if (act) yrow[k] = yprev;
+      }
+    }
+  }
+  // backward substitution, right to left
+  double2 unext = zero;
+  T2* orow = up + static_cast<int64_t>(act ? iy : 0) * G;
+  double2 ny[kCh];
+  auto load_y = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) {
+      const int k = k0 + c;
+      ny[c] = zero;
+      if (act && k < G) ny[c] = yrow[k];
+    }
+  };
+  load_y(((G - 1) / kCh) * kCh);
+  for (int k0 = ((G - 1) / kCh) * kCh; k0 >= 0; k0 -= kCh) {
+    double2 y[kCh];
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) y[c] = ny[c];
+    if (k0 > 0) load_y(k0 - kCh);
+#pragma unroll
+    for (int c = kCh - 1; c >= 0; --c) {
+      const int k = k0 + c;
+      if (k < G) {
+        const double2 uu = (k == G - 1) ? y[c] : zsub(y[c], zmul(cwet_d[k + 1], unext));
+        unext = uu;
+        if (act) {
+          T2 o;
+          o.x = static_cast<P>(uu.x);
+          o.y = static_cast<P>(uu.y);
+          orow[k] = o;
+        }
+      }
+    }
+  }
+}
+
+// y sweep: lane t of block b owns column ix = 64 b + t and walks its rows;
+// the backward pass writes the field and the block's partial field sums
+// (partials[(p * nblk + b) * 8 + k], centre cell at the tail), reduced in
+// block order afterwards.
+template <class P>
+__global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u, double2* ys,
+                                                           int nslice, int G, double cs,
+                                                           const double2* __restrict__ c_d,
+                                                           const double2* __restrict__ cbet_d,
+                                                           const double2* __restrict__ cwet_d,
+                                                           const typename A2<P>::T* src,
+                                                           const int4* rect, int first,
+                                                           int add_src, double* partials) {
+  using T2 = typename A2<P>::T;
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
+  const int nblk = gridDim.y;
+  const int ix = blockIdx.y * kLane + t;
+  const bool act = ix < G;
+  const int ex = t == 0 ? ix - 1 : (t == kLane - 1 ? ix + 1 : -1);
+  const bool eact = ex >= 0 && ex < G;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  T2* up = u + p * plane;
+  double2* yp = ys + p * plane;
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const bool scol = sp && act && ix >= rc.x && ix < rc.z;
+  const int cx = act ? ix : 0, cex = eact ? ex : 0;
+  const double2 zero = make_double2(0.0, 0.0);
+  T2 tz;
+  tz.x = 0;
+  tz.y = 0;
+  T2 nv[kCh], ne[kCh], ns[kCh];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < kCh; ++r) {
+      const int k = k0 + r;
+      nv[r] = tz;
+      ne[r] = tz;
+      ns[r] = tz;
+      if (k < G) {
+        const int64_t rowoff = static_cast<int64_t>(k) * G;
+        if (act) nv[r] = up[rowoff + cx];
+        if (eact) ne[r] = up[rowoff + cex];
+        if (scol && k >= rc.y && k < rc.w) ns[r] = sp[rowoff + cx];
+      }
+    }
+  };
+  load(0);
+  double2 yprev = zero;
+  for (int k0 = 0; k0 < G; k0 += kCh) {
+    double2 v[kCh], e[kCh], sv[kCh];
+#pragma unroll
+    for (int r = 0; r < kCh; ++r) {
+      v[r] = to_d(nv[r]);
+      e[r] = to_d(ne[r]);
+      sv[r] = to_d(ns[r]);
+    }
+    if (k0 + kCh < G) load(k0 + kCh);
+#pragma unroll
+    for (int r = 0; r < kCh; ++r) {
+      const int k = k0 + r;
+      if (k < G) {  // uniform
+        const double2 left = from_lane_below(v[r], e[r]);   // column ix - 1
+        const double2 right = from_lane_above(v[r], e[r]);  // column ix + 1
+        double2 rr = rhs_of(v[r], right, left, cs);
+        if (scol && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
+        yprev = zmul(zsub(rr, zmul(c_d[k], yprev)), cbet_d[k]);
+        if (act) yp[static_cast<int64_t>(k) * G + cx] = yprev;
+      }
+    }
+  }
+  const double shift = -0.5 * static_cast<double>(G - 1);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int64_t icenter = (plane - 1) / 2;
+  double2 unext = zero;
+  double2 ny[kCh];
+  auto load_y = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < kCh; ++r) {
+      const int k = k0 + r;
+      ny[r] = zero;
+      if (act && k < G) ny[r] = yp[static_cast<int64_t>(k) * G + cx];
+    }
+  };
+  load_y(((G - 1) / kCh) * kCh);
+  for (int k0 = ((G - 1) / kCh) * kCh; k0 >= 0; k0 -= kCh) {
+    double2 y[kCh];
+#pragma unroll
+    for (int r = 0; r < kCh; ++r) y[r] = ny[r];
+    if (k0 > 0) load_y(k0 - kCh);
+#pragma unroll
+    for (int r = kCh - 1; r >= 0; --r) {
+      const int k = k0 + r;
+      if (act && k < G) {
+        const double2 uu = (k == G - 1) ? y[r] : zsub(y[r], zmul(cwet_d[k + 1], unext));
+        unext = uu;
+        T2 o;
+        o.x = static_cast<P>(uu.x);
+        o.y = static_cast<P>(uu.y);
+        const int64_t gi = static_cast<int64_t>(k) * G + cx;
+        up[gi] = o;
+        if (partials) {
+          const double re = o.x, im = o.y;
+          const double wei = re * re + im * im;
+          const double dx = ix + shift, dy = k + shift;
+          acc[0] += wei;
+          acc[1] += dx * wei;
+          acc[2] += dx * dx * wei;
+          acc[3] += dy * wei;
+          acc[4] += dy * dy * wei;
+          acc[5] += re;
+          acc[6] += im;
+          if (gi == icenter) {
+            double* c = partials + static_cast<int64_t>(nslice) * nblk * 8 + 2 * p;
+            c[0] = re;
+            c[1] = im;
+          }
+        }
+      }
+    }
+  }
+  if (partials) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      double v = acc[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (t == 0) partials[(static_cast<int64_t>(p) * nblk + blockIdx.y) * 8 + k] = v;
+    }
+  }
+}
+
 template <class P>
 void adi_step(Field* f, bool source_on, bool stats) {
   using T2 = typename A2<P>::T;
   const int G = f->ngrid;
-  require(G <= 360, "ADI solver supports ngrid <= 360 (LDS budget of the x sweep)");
   const int nt = ((G + 63) / 64) * 64;
   const double cs = 0.25 * f->adi_delz / (f->xks * f->dgrid * f->dgrid);
   if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
@@ -305,6 +594,33 @@ void adi_step(Field* f, bool source_on, bool stats) {
   const size_t ldsx = sizeof(double2) * (3 * static_cast<size_t>(G) + (G + 2) * static_cast<size_t>(kB));
   const size_t ldsy = sizeof(double2) * (3 * static_cast<size_t>(G) + 2 * static_cast<size_t>(G + 2));
   const int add = source_on ? 1 : 0;
+  // variant 0: workgroup per plane (its x sweep stages G + 2 rows of kB
+  // columns plus three coefficient tables in LDS, so G <= 360); variant 1:
+  // wave per 64 lines, any G.  Tuned per shape.
+  const int nvar = G <= 360 ? 2 : 1;
+  TuneScope tune(f->ctx, TUNE_ADI, {G, f->nslice, static_cast<int>(f->prec)}, nvar);
+  if (nvar == 1 || tune.variant() == 1) {
+    const int nblk = (G + kLane - 1) / kLane;
+    double* partials = nullptr;
+    if (stats) {
+      const size_t need = static_cast<size_t>(f->nslice) * nblk * 8 + 2 * static_cast<size_t>(f->nslice);
+      if (f->col_partials.size() < need) f->col_partials.alloc(f->ctx, need);
+      partials = f->col_partials.get();
+    }
+    f->ctx->launch(G4H_K_ADI_X, [&] {
+      hipLaunchKernelGGL(adi_x_lane_kernel<P>, dim3(f->nslice, nblk), dim3(kLane), 0,
+                         f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                         f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add);
+    });
+    f->ctx->launch(G4H_K_ADI_Y, [&] {
+      hipLaunchKernelGGL(adi_y_lane_kernel<P>, dim3(f->nslice, nblk), dim3(kLane), 0,
+                         f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                         f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add,
+                         partials);
+    });
+    if (stats) field_reduce_partials(f, partials, nblk);
+    return;
+  }
   f->ctx->launch(G4H_K_ADI_X, [&] {
     hipLaunchKernelGGL(adi_x_kernel<P>, dim3(f->nslice), dim3(nt), ldsx, f->ctx->stream, u,
                        f->adi_y.get(), f->nslice, G, cs, f->adi_c, f->adi_cbet,

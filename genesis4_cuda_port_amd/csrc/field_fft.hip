@@ -499,21 +499,6 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   }
 }
 
-// partials -> plane_stats[p*9 + k] (k < 7 sums, 7..8 centre)
-__global__ void reduce_partials_kernel(const double* partials, int nslice, int ntile,
-                                       double* plane_stats) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nslice) return;
-  double* o = plane_stats + static_cast<int64_t>(p) * G4H_FIELD_RAW;
-  for (int k = 0; k < 7; ++k) {
-    double v = 0.0;
-    for (int t = 0; t < ntile; ++t) v += partials[(static_cast<int64_t>(p) * ntile + t) * 8 + k];
-    o[k] = v;
-  }
-  o[7] = partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p];
-  o[8] = partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1];
-}
-
 // Gather the (rectangle-bounded) source into ring-ordered dense work planes.
 template <class P>
 __global__ void gather_source_kernel(const typename C2<P>::T* src, const int4* rect, int nslice,
@@ -680,12 +665,7 @@ void fft_step(Field* f, bool source_on, bool stats) {
   }
   run_rows<P>(f, data, ROW_PROP, invg);
   run_cols<P>(f, data, COL_PROP, invg, add, partials);
-  if (stats) {
-    f->ctx->launch(G4H_K_OTHER, [&] {
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3((f->nslice + 255) / 256), dim3(256), 0,
-                         f->ctx->stream, partials, f->nslice, ntile, f->plane_stats.get());
-    });
-  }
+  if (stats) field_reduce_partials(f, partials, ntile);
 }
 
 }  // namespace

@@ -37,6 +37,21 @@ __global__ void map_stats_kernel(const double* ps, const int32_t* zeroed, int n,
     raw[static_cast<int64_t>(is) * G4H_FIELD_RAW + k] = z ? 0.0 : ps[static_cast<int64_t>(p) * G4H_FIELD_RAW + k];
 }
 
+// partials -> plane_stats[p*9 + k] (k < 7 sums, 7..8 centre)
+__global__ void reduce_partials_kernel(const double* partials, int nslice, int ntile,
+                                       double* plane_stats) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslice) return;
+  double* o = plane_stats + static_cast<int64_t>(p) * G4H_FIELD_RAW;
+  for (int k = 0; k < 7; ++k) {
+    double v = 0.0;
+    for (int t = 0; t < ntile; ++t) v += partials[(static_cast<int64_t>(p) * ntile + t) * 8 + k];
+    o[k] = v;
+  }
+  o[7] = partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p];
+  o[8] = partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1];
+}
+
 __global__ void clear_flags_kernel(int32_t* zeroed, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) zeroed[i] = 0;
@@ -103,6 +118,13 @@ __global__ void reset_rect_kernel(int4* rect, int4* bbox, int n) {
 }
 
 }  // namespace
+
+void field_reduce_partials(Field* f, const double* partials, int ntile) {
+  f->ctx->launch(G4H_K_OTHER, [&] {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((f->nslice + 255) / 256), dim3(256), 0,
+                       f->ctx->stream, partials, f->nslice, ntile, f->plane_stats.get());
+  });
+}
 
 void ensure_source(Field* f) {
   const int64_t cells = f->cells();

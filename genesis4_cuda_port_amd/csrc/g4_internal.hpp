@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <map>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,6 +43,25 @@ inline void hip_check(hipError_t e, const char* where) {
 #define G4_HIP(expr) ::g4::hip_check((expr), #expr)
 
 void set_last_error(const std::string& m);
+
+// Body of every extern "C" entry point: no exception crosses the ABI; the
+// status is returned and the message kept for g4h_last_error().
+#define G4_API_BEGIN try {
+#define G4_API_END                                \
+  return G4H_OK;                                  \
+  }                                               \
+  catch (const g4::Error& e) {                    \
+    g4::set_last_error(e.what());                 \
+    return e.code;                                \
+  }                                               \
+  catch (const std::bad_alloc&) {                 \
+    g4::set_last_error("host allocation failed"); \
+    return G4H_ERR_OUT_OF_MEMORY;                 \
+  }                                               \
+  catch (const std::exception& e) {               \
+    g4::set_last_error(e.what());                 \
+    return G4H_ERR_DEVICE;                        \
+  }
 
 // Per-context byte accounting so a caller (or a serving loop) can admit work
 // against an HBM budget.
@@ -81,8 +101,39 @@ class DevBuf {
   Context* ctx_ = nullptr;
 };
 
+// Kernel families with interchangeable variants, chosen per shape by the
+// tuner of the context.  Variants of a family produce bit-identical results,
+// so the choice changes only the time a step takes.
+enum TuneFamily : int {
+  TUNE_ADI = 0,    // 0: workgroup per plane (LDS-staged), 1: one wave per 64 lines
+  TUNE_SPLIT = 1,  // workgroups per slice of the particle pass: not tuned, chosen
+                   // from the slice count; pinning sets it (>= 1)
+  TUNE_COUNT
+};
+
+// Autotuner: for each (family, shape) the variants are tried in turn on live
+// launches (kReps times each), timed with events that are read only once
+// they have completed (no host stall), and the fastest is kept for the rest
+// of the context's life.  A family can be pinned to one variant.
+struct Tuner {
+  static constexpr int kReps = 2;
+  struct Trial {
+    int variant;
+    hipEvent_t a, b;
+  };
+  struct Entry {
+    int nvar = 0;
+    int choice = -1;  // decided variant, -1 while measuring
+    int issued = 0;   // trials launched so far
+    std::vector<Trial> trials;
+  };
+  std::map<std::vector<int>, Entry> table;
+  int forced[TUNE_COUNT] = {-1, -1};
+};
+
 struct Context {
   int device = 0;
+  int num_cus = 0;  // compute units of the device (queried on first use)
   hipStream_t stream = nullptr;
   uint64_t bytes = 0;
   // Objects (beams, fields, runs) created from this context.  Destroying the
@@ -121,6 +172,31 @@ struct Context {
     if (pending.size() > 4096) collect();
   }
   void collect();  // synchronises and folds pending events into prof[]
+
+  Tuner tuner;
+  // Variant of `family` for `shape` to launch now; while measuring, *a and *b
+  // are events to record around the launch (else null).
+  int tune_pick(int family, const std::vector<int>& shape, int nvar, hipEvent_t* a,
+                hipEvent_t* b);
+  void tune_release();  // destroys the events of undecided entries
+};
+
+// Records the tuner's events around the launches of one family call.
+class TuneScope {
+ public:
+  TuneScope(Context* ctx, int family, std::vector<int> shape, int nvar) : ctx_(ctx) {
+    variant_ = ctx->tune_pick(family, shape, nvar, &a_, &b_);
+    if (a_) G4_HIP(hipEventRecord(a_, ctx->stream));
+  }
+  ~TuneScope() {
+    if (b_) (void)hipEventRecord(b_, ctx_->stream);
+  }
+  int variant() const { return variant_; }
+
+ private:
+  Context* ctx_;
+  int variant_ = 0;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
 template <class T>
@@ -162,6 +238,11 @@ struct Beam {
   DevBuf<float> xf, yf, pxf, pyf, gf;   // MIXED (gf = gamma - gamma_ref)
   DevBuf<double> theta;                 // both
   DevBuf<double> eloss;                 // per-slice longitudinal field (optional)
+  // scratch of particle passes split over several workgroups per slice
+  DevBuf<double2> split_win;
+  DevBuf<double> split_red;
+  DevBuf<int4> split_fb;
+  DevBuf<int32_t> split_cnt;  // zero between passes
   int max_slice_np = 0;
 };
 

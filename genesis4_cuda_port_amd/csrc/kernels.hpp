@@ -96,6 +96,9 @@ void field_prepare_solver(Field* f, double delz);  // (re)build coefficient tabl
 // pass also writes per-ring-slot field sums into f->plane_stats.
 void field_solve(Field* f, bool source_on, bool stats);
 void field_plane_stats(Field* f);                      // standalone field sums
+// Per-tile partial field sums partials[(p*ntile + t)*8 + k] (k < 7) and the
+// centre cell at partials[nslice*ntile*8 + 2p] -> plane_stats, tiles in order.
+void field_reduce_partials(Field* f, const double* partials, int ntile);
 void field_zero_plane(Field* f, int slot);             // slippage zero fill
 // plane_stats (ring order) -> raw[nslice*9] (logical order), honouring zeroed slots.
 void field_map_stats(Field* f, double* d_raw);

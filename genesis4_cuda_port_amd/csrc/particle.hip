@@ -20,6 +20,14 @@
 // sum: it carries 29 more bits than an FP32 sum, which is what a Kahan
 // compensation term would add, at the cost of one LDS atomic.
 //
+// Few slices (steady-state batches, short time windows) would leave most
+// CUs idle, so such a pass is split: K workgroups per slice (gridDim.y) each
+// take a contiguous part of its particles with the same window placement,
+// write their window and sums to scratch, and the last to arrive (an atomic
+// counter per slice) adds the K windows in part order and writes the plane
+// and the slice's sums.  The previous source rectangle is then zeroed by a
+// pass before, so the fallback atomics of every part land on cleared cells.
+//
 // MIXED arithmetic: x, y, px, py and dgamma = gamma - gamma_ref in FP32,
 // theta in FP64.  The phase equation theta' = xks(1 - 1/beta_par) + xku is
 // evaluated around the reference particle,
@@ -28,9 +36,10 @@
 // with c0, k1, k2 from FP64 on the host; eta is small and well conditioned in
 // FP32, so FP32 keeps ~1e-9 rad/m absolute accuracy where the direct form
 // would cancel 240 - 240 in FP32.
-#include "kernels.hpp"
-
+#include <algorithm>
 #include <cmath>
+
+#include "kernels.hpp"
 
 namespace g4 {
 namespace {
@@ -57,14 +66,20 @@ struct BeamPtr {
   double* th;
 };
 
-// MIXED sine and cosine of an FP64 phase: reduced to [-pi, pi] in FP64
-// (two-term 2*pi), then the hardware v_sin_f32 / v_cos_f32 (absolute error
-// ~1e-6, far inside the 1e-3 mixed-precision budget).
-__device__ __forceinline__ void fsincos(double a, float& s, float& c) {
+// MIXED: an FP64 phase reduced to [-pi, pi] in FP64 (two-term 2*pi) and
+// rounded to FP32.
+__device__ __forceinline__ float reduce_phase(double a) {
   const double n = rint(a * 0.15915494309189535);
   double r = fma(-n, 6.283185307179586, a);
   r = fma(-n, 2.4492935982947064e-16, r);
-  const float rf = static_cast<float>(r);
+  return static_cast<float>(r);
+}
+
+// MIXED sine and cosine of an FP64 phase: the reduced phase through the
+// hardware v_sin_f32 / v_cos_f32 (absolute error ~1e-6, far inside the 1e-3
+// mixed-precision budget).
+__device__ __forceinline__ void fsincos(double a, float& s, float& c) {
+  const float rf = reduce_phase(a);
   s = __sinf(rf);
   c = __cosf(rf);
 }
@@ -329,16 +344,20 @@ struct MixedK {
   float ztemp1, c0, k1, k2, inv_b0, inv_g0;
 };
 
+// The stage phase is the step's initial phase plus dth (the phase advance
+// within the step, |dth| << 1): harmonic h takes sin and cos of its reduced
+// initial phase r0[i] plus h * dth in FP32.
 template <int NF>
 __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int nf, float dg,
-                                      double tthet, float db0, float ez, const float* rr,
-                                      const float* ri, float& k2gg, float& k2pp) {
+                                      const float* r0, float dth, float db0, float ez,
+                                      const float* rr, const float* ri, float& k2gg,
+                                      float& k2pp) {
   float cre = 0.0f, cim = 0.0f;
 #pragma unroll
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
-    float s, c;
-    fsincos(p.f[i].rharm * tthet, s, c);
+    const float a = r0[i] + float(p.f[i].rharm) * dth;
+    const float s = __sinf(a), c = __cosf(a);
     cre += rr[i] * c + ri[i] * s;
     cim += ri[i] * c - rr[i] * s;
   }
@@ -412,12 +431,18 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, double x,
   base_c = bc[0];
 }
 
+// MIXED RK4: the same two-register scheme with the phase advance of the
+// step accumulated in FP32 relative to the FP64 initial phase, which is
+// reduced once per field; theta itself stays FP64 (theta0 + advance).
+// Outputs the reduced initial phase of the first field and the FP64 initial
+// phase for the sincos of the final phase.
 template <int NF>
 __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int slice, float x,
-                                       float y, float px, float py, float& dgamma, double& theta) {
+                                       float y, float px, float py, float& dgamma, double& theta,
+                                       double& base_t, float& base_r) {
   const int nf = NF > 0 ? NF : p.nfield;
   float g = dgamma;
-  double t = theta + p.autophase;
+  const double t0 = theta + p.autophase;
   const float dx = x - float(p.ax), dy = y - float(p.ay);
   const float awm1 = 0.5f * (float(p.kx) * dx * dx + float(p.ky) * dy * dy) + float(p.gradx) * dx +
                      float(p.grady) * dy;
@@ -431,36 +456,45 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
     if (NF == 0 && i >= nf) break;
     gather<float>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
   }
+  float r0[NF > 0 ? NF : kMaxFields];
+#pragma unroll
+  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+    if (NF == 0 && i >= nf) break;
+    r0[i] = reduce_phase(p.f[i].rharm * t0);
+  }
   const float h = float(p.delz);
+  float t = 0.0f;  // phase advance within the step
   float k2gg = 0.0f, k2pp = 0.0f, k3gg, k3pp;
-  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   float stpz = 0.5f * h;
   g += stpz * k2gg;
-  t += double(stpz * k2pp);
+  t += stpz * k2pp;
   k3gg = k2gg;
   k3pp = k2pp;
   k2gg = 0.0f;
   k2pp = 0.0f;
-  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k2gg - k3gg);
-  t += double(stpz * (k2pp - k3pp));
+  t += stpz * (k2pp - k3pp);
   k3gg *= (1.0f / 6.0f);
   k3pp *= (1.0f / 6.0f);
   k2gg *= -0.5f;
   k2pp *= -0.5f;
-  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   stpz = h;
   g += stpz * k2gg;
-  t += double(stpz * k2pp);
+  t += stpz * k2pp;
   k3gg -= k2gg;
   k3pp -= k2pp;
   k2gg *= 2.0f;
   k2pp *= 2.0f;
-  ode32<NF>(p, m, nf, g, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k3gg + k2gg * (1.0f / 6.0f));
-  t += double(stpz * (k3pp + k2pp * (1.0f / 6.0f)));
+  t += stpz * (k3pp + k2pp * (1.0f / 6.0f));
   dgamma = g;
-  theta = t;
+  theta = t0 + double(t);
+  base_t = t0;
+  base_r = r0[0];
 }
 
 __device__ __forceinline__ double gabs(double g, double) { return g; }
@@ -502,6 +536,18 @@ struct PassState {
   P mm[NH > 0 ? 10 : 1];              // xmin, xmax, pxmin, pxmax, ymin, ymax, pymin, pymax, gmin, gmax
 };
 
+// Scratch of a slice split over K workgroups (gridDim.y = K > 1): each
+// part's deposition window, its fallback-cell box and its diagnostic sums,
+// and a per-slice arrival counter (zero between launches).
+constexpr int kSumMax = 12 + 2 * kMaxHarm;
+constexpr int kRedStride = kSumMax + kNMinMax;
+struct SplitScratch {
+  double2* win;  // [nslice][K][window_cap]
+  double* red;   // [nslice][K][kRedStride]
+  int4* fb;      // [nslice][K]
+  int32_t* cnt;  // [nslice]
+};
+
 template <class P, int NF, bool DEP, int NH>
 __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK& mk, int s,
                                         int64_t i, BeamPtr<P> b, P x, P y, P px, P py, P g,
@@ -515,6 +561,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   // FP64: sine and cosine of a phase near which the final phase lies
   // (base_t), known after the push
   double base_t = 0.0, base_s = 0.0, base_c = 1.0;
+  float base_r = 0.0f;  // MIXED: reduced base_t
   bool have_base = false;
   if (st & ST_PUSH) {
     if constexpr (sizeof(P) == 8) {
@@ -524,8 +571,9 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
       have_base = prm.push.f[0].rharm == 1.0;
     } else {
       float gg = g;
-      push32<NF>(prm.push, mk, s, x, y, px, py, gg, th);
+      push32<NF>(prm.push, mk, s, x, y, px, py, gg, th, base_t, base_r);
       g = gg;
+      have_base = prm.push.f[0].rharm == 1.0;
     }
   }
   if (st & ST_R56) th += prm.r56 * gshift(g, g0);
@@ -553,6 +601,10 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
           sincos(th, &a, &c);
         sn1 = a;
         cs1 = c;
+      } else if (have_base) {
+        const float a = base_r + static_cast<float>(th - base_t);
+        sn1 = __sinf(a);
+        cs1 = __cosf(a);
       } else {
         fsincos(th, sn1, cs1);
       }
@@ -684,7 +736,8 @@ __device__ __forceinline__ const ParticleParams& slice_params(const ParticlePara
 template <class P, int NF, bool DEP, int NH, bool TAB>
 __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
                                                             ParticleParams prm_arg,
-                                                            const ParticleParams* __restrict__ tab) {
+                                                            const ParticleParams* __restrict__ tab,
+                                                            SplitScratch sp) {
   const ParticleParams& prm = slice_params<TAB>(prm_arg, tab);
   using T2 = typename V2<P>::T;
   constexpr int NSUM = NH > 0 ? 12 + 2 * NH : 1;  // sums other than the count
@@ -695,7 +748,16 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
 
   const int s = blockIdx.x;
   const int tid = threadIdx.x;
-  const int64_t i0 = off[s], i1 = off[s + 1];
+  // With K = gridDim.y > 1 the slice's particles are split into K contiguous
+  // parts, one per workgroup; the last part to finish combines them.
+  const int K = gridDim.y, kp = blockIdx.y;
+  const int64_t i0s = off[s], i1s = off[s + 1];
+  int64_t i0 = i0s, i1 = i1s;
+  if (K > 1) {
+    const int64_t chunk = (i1s - i0s + K - 1) / K;
+    i0 = min(i1s, i0s + kp * chunk);
+    i1 = min(i1s, i0 + chunk);
+  }
   const double g0 = prm.gamma0;
 
   // ---- deposition set-up -------------------------------------------------
@@ -719,7 +781,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       }
       __syncthreads();
       int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
-      for (int64_t k = i0 + tid; k < i1; k += kThreads) {
+      for (int64_t k = i0s + tid; k < i1s; k += kThreads) {
         int ix, iy;
         P wx, wy;
         if (cell_of<P>(b.x[k], b.y[k], P(prm.dep.gridmax), P(prm.dep.dgrid),
@@ -759,9 +821,10 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       wh = h;
     }
     // zero what the previous deposition left in this slice's source plane
+    // (a split slice has that done by a pass before this one)
     const int4 old = prm.dep.rect[s];
     const int ow = old.z - old.x, oh = old.w - old.y;
-    if (ow > 0 && oh > 0) {
+    if (K == 1 && ow > 0 && oh > 0) {
       for (int k = tid; k < ow * oh; k += kThreads) {
         const int cy = old.y + k / ow, cx = old.x + k % ow;
         T2 z;
@@ -839,14 +902,89 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     have = hnext;
   }
 
+  const int nharm = NH > 0 ? prm.diag.nharm : 0;
+  const int nsum = 12 + 2 * nharm;
+  if constexpr (NH > 0) {
+    // per-wave sums and extremes
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) {
+      if (k < nsum) {
+        const double a = k < 2 ? ps.e[k] : (k < 12 ? static_cast<double>(ps.m[k - 2]) : ps.e[k - 10]);
+        const double v = wave_sum(a);
+        if (lane == 0) s_red[wave][k] = v;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kNMinMax; ++k) {
+      const double m = static_cast<double>(ps.mm[k]);
+      const double v = (k & 1) ? wave_max(m) : wave_min(m);
+      if (lane == 0) s_red[wave][NSUM + k] = v;
+    }
+  }
+  if (K > 1) {
+    // this part's window, fallback box and sums to scratch; the last part
+    // to arrive goes on to combine all K of them in part order
+    __syncthreads();
+    const int64_t part = static_cast<int64_t>(s) * K + kp;
+    if constexpr (DEP) {
+      double2* w = sp.win + part * prm.dep.window_cap;
+      for (int k = tid; k < ww * wh; k += kThreads) w[k] = win[k];
+      if (tid == 0) sp.fb[part] = make_int4(s_fb[0], s_fb[1], s_fb[2], s_fb[3]);
+    }
+    if constexpr (NH > 0) {
+      double* r = sp.red + part * kRedStride;
+      if (tid < nsum) {
+        double v = 0.0;
+        for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+        r[tid] = v;
+      } else if (tid >= 128 && tid < 128 + kNMinMax) {
+        const int k = tid - 128;
+        double v = s_red[0][NSUM + k];
+        for (int w = 1; w < kWaves; ++w)
+          v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
+        r[kSumMax + k] = v;
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ int s_last;
+    if (tid == 0) s_last = atomicAdd(&sp.cnt[s], 1) == K - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (tid == 0) {
+      if constexpr (DEP) {
+        for (int q = 0; q < K; ++q) {
+          const int4 f = sp.fb[static_cast<int64_t>(s) * K + q];
+          s_fb[0] = min(s_fb[0], f.x);
+          s_fb[1] = min(s_fb[1], f.y);
+          s_fb[2] = max(s_fb[2], f.z);
+          s_fb[3] = max(s_fb[3], f.w);
+        }
+      }
+      sp.cnt[s] = 0;
+    }
+    __syncthreads();
+  }
+
   if constexpr (DEP) {
     __syncthreads();
     // write the window with plain stores; its non-zero cells give the
     // footprint for the next step's window
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+    const double2* parts = sp.win + static_cast<int64_t>(s) * K * prm.dep.window_cap;
     for (int k = tid; k < ww * wh; k += kThreads) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
-      const double2 v = win[k];
+      double2 v = win[k];
+      if (K > 1) {
+        v = parts[k];
+        for (int q = 1; q < K; ++q) {
+          const double2 a = parts[static_cast<int64_t>(q) * prm.dep.window_cap + k];
+          v.x += a.x;
+          v.y += a.y;
+        }
+      }
       T2 o;
       o.x = static_cast<P>(v.x);
       o.y = static_cast<P>(v.y);
@@ -888,39 +1026,35 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   }
 
   if constexpr (NH > 0) {
-    const int nharm = prm.diag.nharm;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int nsum = 12 + 2 * nharm;
-#pragma unroll
-    for (int k = 0; k < NSUM; ++k) {
-      if (k < nsum) {
-        const double a = k < 2 ? ps.e[k] : (k < 12 ? static_cast<double>(ps.m[k - 2]) : ps.e[k - 10]);
-        const double v = wave_sum(a);
-        if (lane == 0) s_red[wave][k] = v;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kNMinMax; ++k) {
-      const double m = static_cast<double>(ps.mm[k]);
-      const double v = (k & 1) ? wave_max(m) : wave_min(m);
-      if (lane == 0) s_red[wave][NSUM + k] = v;
-    }
     __syncthreads();
+    const double* parts = sp.red + static_cast<int64_t>(s) * K * kRedStride;
     double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
     // raw layout: 12 sums (G4H_B_G1..G4H_B_YPY), the count, 10 min/max, then
     // (cos, sin) sums per harmonic
     if (tid < nsum) {
       double v = 0.0;
-      for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+      if (K == 1) {
+        for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+      } else {
+        for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
+      }
       const int dst = tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12);
       out[dst] = v;
     } else if (tid == 64) {
-      out[G4H_B_COUNT] = static_cast<double>(i1 - i0);
+      out[G4H_B_COUNT] = static_cast<double>(i1s - i0s);
     } else if (tid >= 128 && tid < 128 + kNMinMax) {
       const int k = tid - 128;
-      double v = s_red[0][NSUM + k];
-      for (int w = 1; w < kWaves; ++w)
-        v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
+      double v;
+      if (K == 1) {
+        v = s_red[0][NSUM + k];
+        for (int w = 1; w < kWaves; ++w)
+          v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
+      } else {
+        v = parts[kSumMax + k];
+        for (int q = 1; q < K; ++q)
+          v = (k & 1) ? fmax(v, parts[q * kRedStride + kSumMax + k])
+                      : fmin(v, parts[q * kRedStride + kSumMax + k]);
+      }
       if (sizeof(P) == 4 && k >= 8) v += g0;  // MIXED energy extremes were shifted
       // order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
       out[G4H_B_XMIN + k] = v;
@@ -940,13 +1074,73 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
                 : -1;
 }
 
+// Zero the cells the previous deposition left in each slice's source plane
+// (the source rectangle) before a split pass deposits into it.
+template <class P, bool TAB>
+__global__ void zero_rect_kernel(ParticleParams prm_arg, const ParticleParams* __restrict__ tab) {
+  const ParticleParams& prm = slice_params<TAB>(prm_arg, tab);
+  using T2 = typename V2<P>::T;
+  const int s = blockIdx.x;
+  const int ng = prm.dep.ngrid;
+  T2* src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
+  const int4 old = prm.dep.rect[s];
+  const int ow = old.z - old.x, oh = old.w - old.y;
+  if (ow <= 0 || oh <= 0) return;
+  for (int k = threadIdx.x; k < ow * oh; k += blockDim.x) {
+    const int cy = old.y + k / ow, cx = old.x + k % ow;
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+    src[cx + static_cast<int64_t>(cy) * ng] = z;
+  }
+}
+
+// Workgroups per slice: a pass over few slices is split so that about two
+// workgroups per CU are in flight, keeping at least 8 iterations of the
+// particle loop per workgroup (or as pinned on the context).
+int split_of(Context* ctx, const Beam* b) {
+  const int forced = ctx->tuner.forced[TUNE_SPLIT];
+  if (forced >= 1) return std::min(forced, 64);
+  if (ctx->num_cus == 0) {
+    int n = 0;
+    G4_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ctx->num_cus = n > 0 ? n : 1;
+  }
+  const int target = 2 * ctx->num_cus;
+  if (b->nslice >= target) return 1;
+  int k = (target + b->nslice - 1) / b->nslice;
+  k = std::min(k, b->max_slice_np / (8 * kThreads));
+  return std::max(1, std::min(k, 64));
+}
+
 template <class P, int NF, bool DEP, int NH, bool TAB>
 void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
               const ParticleParams* tab) {
   const size_t lds = DEP ? static_cast<size_t>(p.dep.window_cap) * sizeof(double2) : 0;
+  const int K = split_of(ctx, b);
+  SplitScratch sp{nullptr, nullptr, nullptr, nullptr};
+  if (K > 1) {
+    const size_t parts = static_cast<size_t>(b->nslice) * K;
+    const size_t nwin = DEP ? parts * static_cast<size_t>(p.dep.window_cap) : 0;
+    if (b->split_win.size() < nwin) b->split_win.alloc(ctx, nwin);
+    if (b->split_red.size() < parts * kRedStride) b->split_red.alloc(ctx, parts * kRedStride);
+    if (b->split_fb.size() < parts) b->split_fb.alloc(ctx, parts);
+    if (b->split_cnt.size() < static_cast<size_t>(b->nslice)) {
+      b->split_cnt.alloc(ctx, static_cast<size_t>(b->nslice));
+      G4_HIP(hipMemsetAsync(b->split_cnt.get(), 0, sizeof(int32_t) * b->nslice, ctx->stream));
+    }
+    sp = SplitScratch{b->split_win.get(), b->split_red.get(), b->split_fb.get(),
+                      b->split_cnt.get()};
+    if (DEP) {
+      ctx->launch(G4H_K_OTHER, [&] {
+        hipLaunchKernelGGL((zero_rect_kernel<P, TAB>), dim3(b->nslice), dim3(256), 0,
+                           ctx->stream, p, tab);
+      });
+    }
+  }
   ctx->launch(G4H_K_PARTICLE, [&] {
-    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH, TAB>), dim3(b->nslice), dim3(kThreads),
-                       lds, ctx->stream, bp, b->off.get(), p, tab);
+    hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH, TAB>), dim3(b->nslice, K),
+                       dim3(kThreads), lds, ctx->stream, bp, b->off.get(), p, tab, sp);
   });
 }
 

@@ -84,19 +84,6 @@ struct g4h_run {
   }
 };
 
-#define G4_API_BEGIN try {
-#define G4_API_END                     \
-  return G4H_OK;                       \
-  }                                    \
-  catch (const g4::Error& e) {         \
-    g4::set_last_error(e.what());      \
-    return e.code;                     \
-  }                                    \
-  catch (const std::exception& e) {    \
-    g4::set_last_error(e.what());      \
-    return G4H_ERR_DEVICE;             \
-  }
-
 extern "C" {
 
 g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,

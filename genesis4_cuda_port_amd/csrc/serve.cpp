@@ -332,19 +332,6 @@ void g4h_server::loop() {
   }
 }
 
-#define G4_API_BEGIN try {
-#define G4_API_END                     \
-  return G4H_OK;                       \
-  }                                    \
-  catch (const g4::Error& e) {         \
-    g4::set_last_error(e.what());      \
-    return e.code;                     \
-  }                                    \
-  catch (const std::exception& e) {    \
-    g4::set_last_error(e.what());      \
-    return G4H_ERR_DEVICE;             \
-  }
-
 extern "C" {
 
 g4h_status g4h_server_create(int device, uint64_t hbm_budget, int nstreams, g4h_server** out) {

@@ -1,0 +1,103 @@
+// tune.cpp - the per-context autotuner of kernel variants.
+//
+// A family's variants compute the same thing in the same arithmetic order
+// and differ in how the work is spread over the GPU (workgroup per plane or
+// wave per line block, tile widths).  Which is faster depends on the shape:
+// many planes fill the GPU either way, a handful of planes leaves it idle
+// unless the lines of one plane are spread out.  The tuner measures instead
+// of guessing: the first kReps * nvar calls of a (family, shape) run the
+// variants in turn between two events; once every trial's end event has
+// completed (checked with hipEventQuery, so the host never waits) the
+// variant with the smallest best time is fixed for that shape.
+#include <algorithm>
+#include <limits>
+
+#include "kernels.hpp"
+
+namespace g4 {
+
+int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipEvent_t* a,
+                       hipEvent_t* b) {
+  *a = nullptr;
+  *b = nullptr;
+  if (family < 0 || family >= TUNE_COUNT || nvar <= 1) return 0;
+  const int forced = tuner.forced[family];
+  if (forced >= 0) return std::min(forced, nvar - 1);
+  std::vector<int> key;
+  key.reserve(shape.size() + 1);
+  key.push_back(family);
+  key.insert(key.end(), shape.begin(), shape.end());
+  Tuner::Entry& e = tuner.table[key];
+  if (e.nvar == 0) e.nvar = nvar;
+  if (e.choice >= 0) return e.choice;
+  const int total = Tuner::kReps * e.nvar;
+  if (e.issued < total) {
+    // next trial: variants in turn, so every variant sees warm and cold caches
+    Tuner::Trial t{e.issued % e.nvar, nullptr, nullptr};
+    G4_HIP(hipEventCreate(&t.a));
+    G4_HIP(hipEventCreate(&t.b));
+    e.trials.push_back(t);
+    ++e.issued;
+    *a = t.a;
+    *b = t.b;
+    return t.variant;
+  }
+  // all trials issued: decide once they have all completed
+  for (const Tuner::Trial& t : e.trials) {
+    const hipError_t q = hipEventQuery(t.b);
+    if (q == hipErrorNotReady) return 0;  // provisional: variant 0
+    G4_HIP(q);
+  }
+  std::vector<float> best(static_cast<size_t>(e.nvar), std::numeric_limits<float>::max());
+  for (const Tuner::Trial& t : e.trials) {
+    float ms = 0.f;
+    G4_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+    best[static_cast<size_t>(t.variant)] = std::min(best[static_cast<size_t>(t.variant)], ms);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  e.trials.clear();
+  e.choice = static_cast<int>(std::min_element(best.begin(), best.end()) - best.begin());
+  return e.choice;
+}
+
+void Context::tune_release() {
+  for (auto& kv : tuner.table) {
+    for (const Tuner::Trial& t : kv.second.trials) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    kv.second.trials.clear();
+  }
+}
+
+}  // namespace g4
+
+using namespace g4;
+
+extern "C" {
+
+g4h_status g4h_context_set_variant(g4h_context* ctx, int family, int variant) {
+  G4_API_BEGIN
+  require(ctx, "ctx is null");
+  require(family >= 0 && family < TUNE_COUNT, "unknown kernel family");
+  require(variant >= -1, "variant must be >= -1 (-1: autotune)");
+  require(family != TUNE_SPLIT || variant != 0, "workgroups per slice must be >= 1");
+  ctx->tuner.forced[family] = variant;
+  G4_API_END
+}
+
+g4h_status g4h_context_get_variant(g4h_context* ctx, int family, const int* shape, int nshape,
+                                   int* variant) {
+  G4_API_BEGIN
+  require(ctx && variant, "null argument");
+  require(family >= 0 && family < TUNE_COUNT, "unknown kernel family");
+  require(nshape >= 0 && (nshape == 0 || shape), "invalid shape");
+  std::vector<int> key{family};
+  for (int i = 0; i < nshape; ++i) key.push_back(shape[i]);
+  auto it = ctx->tuner.table.find(key);
+  *variant = (it == ctx->tuner.table.end()) ? -1 : it->second.choice;
+  G4_API_END
+}
+
+}  // extern "C"

@@ -55,6 +55,23 @@ g4h_status g4h_context_synchronize(g4h_context* ctx);
 /* Device bytes currently held by objects of this context. */
 g4h_status g4h_context_memory(const g4h_context* ctx, uint64_t* bytes_in_use);
 
+/* Kernel variants and the autotuner.  Some kernel families have variants
+ * that compute bit-identical results with different work decompositions;
+ * per shape the context tries each on its first launches and keeps the
+ * fastest.  No counterpart in the reference (its kernels have one launch
+ * shape each).  Families: */
+#define G4H_TUNE_ADI 0   /* 0: workgroup per plane, 1: wave per 64 lines */
+#define G4H_TUNE_SPLIT 1 /* workgroups per slice of the particle pass (>= 1; by
+                            default chosen from the slice count so that few
+                            slices still fill the GPU; results agree to
+                            round-off, not bit for bit) */
+/* Pin `family` to `variant` on this context (-1: autotune, the default). */
+g4h_status g4h_context_set_variant(g4h_context* ctx, int family, int variant);
+/* Decided variant of `family` for a shape ({ngrid, nslice, precision} for
+ * G4H_TUNE_ADI), -1 while still measuring or never launched. */
+g4h_status g4h_context_get_variant(g4h_context* ctx, int family, const int* shape, int nshape,
+                                   int* variant);
+
 /* ------------------------------------------------------------------------ */
 /* Beam: device-resident SoA (replaces Genesis4BeamSoA, include/              */
 /* Genesis4BeamSoA.h:7-21, and Beam::init_beamSoA/pack_beam_to_soa/          */
