@@ -301,6 +301,15 @@ struct RegCfg {
   static constexpr int kThreads = 64 * kWaves;
   // column tile: one column per line slot of the workgroup
   static constexpr int kTile = kWaves * S::LPW < G ? kWaves * S::LPW : G;
+  // the column pass stages its tile (G rows of kTile + 1 padded columns)
+  // through LDS when every line slot holds a column, a tile row is at least
+  // 64 bytes and the staged copy covers the G rows in whole instructions
+  static constexpr bool kStage = kWaves * S::LPW <= G && kThreads % kTile == 0 &&
+                                 G % (kThreads / kTile) == 0 &&
+                                 kTile * 2 * static_cast<int>(sizeof(P)) >= 64;
+  static constexpr int kBuf = kWaves * S::LPW * S::BUF;
+  static constexpr int kColBuf =
+      kStage && G * (kTile + 1) > kBuf ? G * (kTile + 1) : kBuf;
 };
 
 template <class T2>
@@ -394,7 +403,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   constexpr int ntile = G / C::kTile;
   __shared__ T2 tw[G];
   __shared__ T2 mu[G];
-  __shared__ T2 buf[C::kWaves * S::LPW * S::BUF];
+  __shared__ T2 buf[C::kColBuf];
   __shared__ double red[C::kWaves][8];
   load_table(tw, tw_d, G);
   if (mode == COL_PROP) load_table(mu, mult_d, G);
@@ -408,13 +417,35 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   const bool valid = cslot < C::kTile;
   T2* lbuf = buf + cslot * S::BUF;
   T2* plane = u + static_cast<int64_t>(p) * G * G;
+  // Staged tile: the workgroup's kTile columns of all G rows pass through
+  // LDS (reusing the transpose buffer), so each load and store instruction
+  // covers whole row segments of the tile instead of L rows x LPW columns.
+  constexpr int TW = C::kTile, TP = TW + 1;
+  constexpr bool kStage = C::kStage;
+  constexpr int RPI = C::kThreads / TW;  // rows per instruction of the staged copy
+  const int c0 = tile * TW;
   T2 v[R];
-  fftr::static_for<0, R>([&](auto ic) {
-    constexpr int r = decltype(ic)::value;
-    v[r].x = 0;
-    v[r].y = 0;
-    if (valid) v[r] = plane[static_cast<int64_t>(t + L * r) * G + c];
-  });
+  if constexpr (kStage) {
+    const int sc_col = threadIdx.x % TW, sc_row = threadIdx.x / TW;
+    fftr::static_for<0, G / RPI>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int row = i * RPI + sc_row;
+      buf[row * TP + sc_col] = plane[static_cast<int64_t>(row) * G + c0 + sc_col];
+    });
+    __syncthreads();
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      v[r] = buf[(t + L * r) * TP + cslot];
+    });
+    __syncthreads();
+  } else {
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      v[r].x = 0;
+      v[r].y = 0;
+      if (valid) v[r] = plane[static_cast<int64_t>(t + L * r) * G + c];
+    });
+  }
   fftr::forward<T2, G>(v, t, tw, lbuf);
   if (mode == COL_PROP) {
     fftr::static_for<0, R>([&](auto ic) {
@@ -445,7 +476,52 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   const int64_t icenter = (static_cast<int64_t>(G) * G - 1) / 2;
   const bool src_col = sp && c >= rc.x && c < rc.z;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (valid) {
+  if constexpr (kStage) {
+    __syncthreads();  // the inverse transform's LDS use is over
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      buf[(t + L * r) * TP + cslot] = v[r];
+    });
+    __syncthreads();
+    const int sc_col = threadIdx.x % TW, sc_row = threadIdx.x / TW;
+    const int col = c0 + sc_col;
+    const bool scol = sp && col >= rc.x && col < rc.z;
+    const double dx = col + shift;
+    fftr::static_for<0, G / RPI>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int row = i * RPI + sc_row;
+      const int64_t gi = static_cast<int64_t>(row) * G + col;
+      T2 o = buf[row * TP + sc_col];
+      o.x *= sc;
+      o.y *= sc;
+      if (scol && row >= rc.y && row < rc.w) {
+        const T2 sv = sp[gi];
+        o.x += P(2) * sv.x;
+        o.y += P(2) * sv.y;
+      } else if (wp) {
+        const double2 wv = wp[gi];
+        o.x += static_cast<P>(wv.x);
+        o.y += static_cast<P>(wv.y);
+      }
+      plane[gi] = o;
+      if (partials) {
+        const double re = o.x, im = o.y;
+        const double wei = re * re + im * im;
+        const double dy = row + shift;
+        acc[0] += wei;
+        acc[1] += dx * wei;
+        acc[2] += dx * dx * wei;
+        acc[3] += dy * wei;
+        acc[4] += dy * dy * wei;
+        acc[5] += re;
+        acc[6] += im;
+        if (gi == icenter) {
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+        }
+      }
+    });
+  } else if (valid) {
     const double dx = c + shift;
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
