@@ -42,6 +42,47 @@ void Context::collect() {
   pending.clear();
 }
 
+const void* Context::param_push(const void* data, size_t bytes) {
+  require(bytes <= kParamBytes, "parameter block too large");
+  if (!param_host) {
+    void* h = nullptr;
+    G4_HIP(hipHostMalloc(&h, kParamSlots * kParamBytes, hipHostMallocDefault));
+    param_host = static_cast<char*>(h);
+    void* d = nullptr;
+    G4_HIP(hipMalloc(&d, kParamSlots * kParamBytes));
+    param_dev = static_cast<char*>(d);
+    this->bytes += kParamSlots * kParamBytes;
+    for (hipEvent_t& e : param_ev) G4_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const int k = param_next;
+  param_next = (k + 1) % kParamSlots;
+  if (param_pending[k]) {
+    G4_HIP(hipEventSynchronize(param_ev[k]));
+    param_pending[k] = false;
+  }
+  char* h = param_host + static_cast<size_t>(k) * kParamBytes;
+  char* d = param_dev + static_cast<size_t>(k) * kParamBytes;
+  std::memcpy(h, data, bytes);
+  G4_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+  param_last = k;
+  return d;
+}
+
+void Context::param_commit() {
+  if (param_last < 0) return;
+  G4_HIP(hipEventRecord(param_ev[param_last], stream));
+  param_pending[param_last] = true;
+  param_last = -1;
+}
+
+void Context::param_release() {
+  for (hipEvent_t e : param_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (param_host) (void)hipHostFree(param_host);
+  if (param_dev) (void)hipFree(param_dev);
+  param_host = param_dev = nullptr;
+}
+
 // ---- parameter blocks ------------------------------------------------------
 
 // TrackBeamCUDA::track strengths and maps (TrackBeamCUDA.cpp:163-188) and the
@@ -250,6 +291,7 @@ static void release_context(g4h_context* ctx) {
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   ctx->tune_release();
+  ctx->param_release();
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

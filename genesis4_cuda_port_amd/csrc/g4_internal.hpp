@@ -173,6 +173,23 @@ struct Context {
   }
   void collect();  // synchronises and folds pending events into prof[]
 
+  // Parameter blocks that kernels read through a pointer (rather than as
+  // by-value arguments, which the compiler keeps live in scalar registers
+  // and spills): staged through pinned host memory into a device ring.
+  // param_push copies a block into the next slot and returns its device
+  // address; param_commit marks the slot in use until the work enqueued so
+  // far on the stream has run.
+  static constexpr int kParamSlots = 64;
+  static constexpr size_t kParamBytes = 4096;
+  char* param_host = nullptr;
+  char* param_dev = nullptr;
+  hipEvent_t param_ev[kParamSlots] = {};
+  bool param_pending[kParamSlots] = {};
+  int param_next = 0, param_last = -1;
+  const void* param_push(const void* data, size_t bytes);
+  void param_commit();
+  void param_release();
+
   Tuner tuner;
   // Variant of `family` for `shape` to launch now; while measuring, *a and *b
   // are events to record around the launch (else null).

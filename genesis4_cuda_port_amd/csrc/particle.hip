@@ -529,6 +529,59 @@ __device__ __forceinline__ double wave_max(double v) {
 // mode and, in MIXED mode, FP32 over the thread's own particles (a slice's
 // particles / kThreads of them); every reduction across threads, waves and
 // slices is FP64.
+// quad_perm DPP: lane i takes lane i ^ 1 / i ^ 2 of its quad
+__device__ __forceinline__ int quad_xor1(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int quad_xor2(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float quad_xor1(float v) {
+  return __builtin_bit_cast(float, quad_xor1(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float quad_xor2(float v) {
+  return __builtin_bit_cast(float, quad_xor2(__builtin_bit_cast(int, v)));
+}
+
+// One particle's source terms into the 2x2 cells from (ix, iy): LDS window
+// cells with ds_add_f64, cells outside the window with global atomics (and
+// the box of such cells for the next footprint).
+template <class P>
+__device__ __forceinline__ void deposit_one(P cre, P cim, const P* w, int ix, int iy, int wx0,
+                                            int wy0, int ww, int wh, int ng,
+                                            double2* win, typename V2<P>::T* src, int* s_fb) {
+  const int lx0 = ix - wx0, ly0 = iy - wy0;
+  if (lx0 >= 0 && lx0 + 1 < ww && ly0 >= 0 && ly0 + 1 < wh) {
+    // the whole 2x2 block lies in the window (the common case)
+    double* a = &win[lx0 + ly0 * ww].x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double* ak = a + 2 * ((k & 1) + (k >> 1) * ww);
+      lds_add(ak, static_cast<double>(cre * w[k]));
+      lds_add(ak + 1, static_cast<double>(cim * w[k]));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cx = ix + (k & 1), cy = iy + (k >> 1);
+      const int lx = cx - wx0, ly = cy - wy0;
+      if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+        double* a = &win[lx + ly * ww].x;
+        lds_add(a, static_cast<double>(cre * w[k]));
+        lds_add(a + 1, static_cast<double>(cim * w[k]));
+      } else {
+        P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
+        glob_add(a, cre * w[k]);
+        glob_add(a + 1, cim * w[k]);
+        atomicMin(&s_fb[0], cx);
+        atomicMin(&s_fb[1], cy);
+        atomicMax(&s_fb[2], cx);
+        atomicMax(&s_fb[3], cy);
+      }
+    }
+  }
+}
+
 template <class P, int NH>
 struct PassState {
   double e[NH > 0 ? 2 + 2 * NH : 1];  // dgamma, dgamma^2, then (cos, sin) per harmonic
@@ -612,7 +665,69 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
     }
   };
 
-  if constexpr (DEP) {
+  if constexpr (DEP && sizeof(P) == 4) {
+    if (scl != P(0)) {
+      // MIXED: every lane computes its contribution (zero weight when the
+      // particle is off the grid), so that the four lanes of a quad (the
+      // nbins = 4 mirrored particles of a beamlet, which share their
+      // transverse position unless the beam has been disturbed) can combine
+      // theirs when they fall on the same 2x2 block of the window: one lane
+      // then adds the quad's sums, a quarter of the LDS atomics and none of
+      // the same-address conflicts of four lanes adding to one cell.
+      const DepParams& d = prm.dep;
+      int ix = 0, iy = 0;
+      P wx = 0, wy = 0;
+      const bool in = cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), P(d.inv_dgrid), ng, ix, iy, wx, wy);
+      const P dx = x - P(d.ax), dy = y - P(d.ay);
+      const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
+                     P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
+      P sn, cs;
+      if (d.harm == 1) {
+        phase1();
+        sn = sn1;
+        cs = cs1;
+      } else {
+        fsincos(static_cast<double>(d.harm) * th, sn, cs);
+      }
+      const P part = in ? psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g) : P(0);
+      const P cre = sn * part;
+      const P cim = cs * part;
+      const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
+      const int lx0 = ix - wx0, ly0 = iy - wy0;
+      const bool fast = in && lx0 >= 0 && lx0 + 1 < ww && ly0 >= 0 && ly0 + 1 < wh;
+      bool done = false;
+      if (__builtin_amdgcn_read_exec() == ~0ull) {  // whole wave here: quads are complete
+        const int key = fast ? lx0 + ly0 * ww : -1 - static_cast<int>(threadIdx.x & 63);
+        int same = (key == quad_xor1(key) && key == quad_xor2(key)) ? 1 : 0;
+        same &= quad_xor1(same) & quad_xor2(same);
+        if (same) {
+          P v[8];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            v[2 * k] = cre * w[k];
+            v[2 * k + 1] = cim * w[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            v[k] += quad_xor1(v[k]);
+            v[k] += quad_xor2(v[k]);
+          }
+          if ((threadIdx.x & 3) == 0) {
+            double* a = &win[lx0 + ly0 * ww].x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              double* ak = a + 2 * ((k & 1) + (k >> 1) * ww);
+              lds_add(ak, static_cast<double>(v[2 * k]));
+              lds_add(ak + 1, static_cast<double>(v[2 * k + 1]));
+            }
+          }
+          done = true;
+        }
+      }
+      if (!done && in) deposit_one<P>(cre, cim, w, ix, iy, wx0, wy0, ww, wh, ng, win, src, s_fb);
+    }
+  }
+  if constexpr (DEP && sizeof(P) == 8) {
     if (scl != P(0)) {
       const DepParams& d = prm.dep;
       int ix, iy;
@@ -638,36 +753,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
         const P cre = sn * part;
         const P cim = cs * part;
         const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
-        const int lx0 = ix - wx0, ly0 = iy - wy0;
-        if (lx0 >= 0 && lx0 + 1 < ww && ly0 >= 0 && ly0 + 1 < wh) {
-          // the whole 2x2 block lies in the window (the common case)
-          double* a = &win[lx0 + ly0 * ww].x;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            double* ak = a + 2 * ((k & 1) + (k >> 1) * ww);
-            lds_add(ak, static_cast<double>(cre * w[k]));
-            lds_add(ak + 1, static_cast<double>(cim * w[k]));
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int cx = ix + (k & 1), cy = iy + (k >> 1);
-            const int lx = cx - wx0, ly = cy - wy0;
-            if (lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
-              double* a = &win[lx + ly * ww].x;
-              lds_add(a, static_cast<double>(cre * w[k]));
-              lds_add(a + 1, static_cast<double>(cim * w[k]));
-            } else {
-              P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
-              glob_add(a, cre * w[k]);
-              glob_add(a + 1, cim * w[k]);
-              atomicMin(&s_fb[0], cx);
-              atomicMin(&s_fb[1], cy);
-              atomicMax(&s_fb[2], cx);
-              atomicMax(&s_fb[3], cy);
-            }
-          }
-        }
+        deposit_one<P>(cre, cim, w, ix, iy, wx0, wy0, ww, wh, ng, win, src, s_fb);
       }
     }
   }
@@ -721,16 +807,21 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   }
 }
 
-// Parameters of this workgroup's slice: the launch's by-value block, or
-// with TAB the slice's entry of a device table (batches of independent runs
-// whose per-step parameters differ from run to run).
-template <bool TAB>
-__device__ __forceinline__ const ParticleParams& slice_params(const ParticleParams& a,
+// Parameters of this workgroup's slice.  With TAB the slice's entry of a
+// device table (batches of independent runs whose per-step parameters differ
+// from run to run); else one block for all slices: by value in MIXED, and in
+// FP64 a device copy read through a pointer, whose fields are then loaded
+// where they are used instead of being held (and spilled) in scalar
+// registers across the particle loop of the larger FP64 kernel.
+template <bool TAB, class P>
+__device__ __forceinline__ const ParticleParams& slice_params(const ParticleParams& arg,
                                                              const ParticleParams* tab) {
   if constexpr (TAB)
     return tab[blockIdx.x];
+  else if constexpr (sizeof(P) == 8)
+    return tab[0];
   else
-    return a;
+    return arg;
 }
 
 template <class P, int NF, bool DEP, int NH, bool TAB>
@@ -738,7 +829,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
                                                             ParticleParams prm_arg,
                                                             const ParticleParams* __restrict__ tab,
                                                             SplitScratch sp) {
-  const ParticleParams& prm = slice_params<TAB>(prm_arg, tab);
+  const ParticleParams& prm = slice_params<TAB, P>(prm_arg, tab);
   using T2 = typename V2<P>::T;
   constexpr int NSUM = NH > 0 ? 12 + 2 * NH : 1;  // sums other than the count
   extern __shared__ double2 win[];
@@ -1078,7 +1169,7 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
 // (the source rectangle) before a split pass deposits into it.
 template <class P, bool TAB>
 __global__ void zero_rect_kernel(ParticleParams prm_arg, const ParticleParams* __restrict__ tab) {
-  const ParticleParams& prm = slice_params<TAB>(prm_arg, tab);
+  const ParticleParams& prm = slice_params<TAB, P>(prm_arg, tab);
   using T2 = typename V2<P>::T;
   const int s = blockIdx.x;
   const int ng = prm.dep.ngrid;
@@ -1118,6 +1209,8 @@ void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
               const ParticleParams* tab) {
   const size_t lds = DEP ? static_cast<size_t>(p.dep.window_cap) * sizeof(double2) : 0;
   const int K = split_of(ctx, b);
+  if (!tab && sizeof(P) == 8)
+    tab = static_cast<const ParticleParams*>(ctx->param_push(&p, sizeof(p)));
   SplitScratch sp{nullptr, nullptr, nullptr, nullptr};
   if (K > 1) {
     const size_t parts = static_cast<size_t>(b->nslice) * K;
@@ -1142,6 +1235,7 @@ void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
     hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH, TAB>), dim3(b->nslice, K),
                        dim3(kThreads), lds, ctx->stream, bp, b->off.get(), p, tab, sp);
   });
+  ctx->param_commit();
 }
 
 template <class P, int NF, bool DEP, bool TAB>
