@@ -542,6 +542,18 @@ __device__ __forceinline__ float quad_xor1(float v) {
 __device__ __forceinline__ float quad_xor2(float v) {
   return __builtin_bit_cast(float, quad_xor2(__builtin_bit_cast(int, v)));
 }
+__device__ __forceinline__ double quad_xor1(double v) {
+  int2 a = __builtin_bit_cast(int2, v);
+  a.x = quad_xor1(a.x);
+  a.y = quad_xor1(a.y);
+  return __builtin_bit_cast(double, a);
+}
+__device__ __forceinline__ double quad_xor2(double v) {
+  int2 a = __builtin_bit_cast(int2, v);
+  a.x = quad_xor2(a.x);
+  a.y = quad_xor2(a.y);
+  return __builtin_bit_cast(double, a);
+}
 
 // One particle's source terms into the 2x2 cells from (ix, iy): LDS window
 // cells with ds_add_f64, cells outside the window with global atomics (and
@@ -665,9 +677,9 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
     }
   };
 
-  if constexpr (DEP && sizeof(P) == 4) {
+  if constexpr (DEP) {
     if (scl != P(0)) {
-      // MIXED: every lane computes its contribution (zero weight when the
+      // Every lane computes its contribution (zero weight when the
       // particle is off the grid), so that the four lanes of a quad (the
       // nbins = 4 mirrored particles of a beamlet, which share their
       // transverse position unless the beam has been disturbed) can combine
@@ -686,10 +698,18 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
         phase1();
         sn = sn1;
         cs = cs1;
+      } else if constexpr (sizeof(P) == 8) {
+        sincos(static_cast<double>(d.harm) * th, &sn, &cs);
       } else {
         fsincos(static_cast<double>(d.harm) * th, sn, cs);
       }
-      const P part = in ? psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g) : P(0);
+      P part = P(0);
+      if (in) {
+        if constexpr (sizeof(P) == 8)
+          part = sqrt(faw2) * scl / g;
+        else
+          part = psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g);
+      }
       const P cre = sn * part;
       const P cim = cs * part;
       const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
@@ -727,37 +747,6 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
       if (!done && in) deposit_one<P>(cre, cim, w, ix, iy, wx0, wy0, ww, wh, ng, win, src, s_fb);
     }
   }
-  if constexpr (DEP && sizeof(P) == 8) {
-    if (scl != P(0)) {
-      const DepParams& d = prm.dep;
-      int ix, iy;
-      P wx, wy;
-      if (cell_of<P>(x, y, P(d.gridmax), P(d.dgrid), P(d.inv_dgrid), ng, ix, iy, wx, wy)) {
-        const P dx = x - P(d.ax), dy = y - P(d.ay);
-        const P faw2 = P(1) + P(d.kx) * dx * dx + P(d.ky) * dy * dy +
-                       P(2) * (P(d.gradx) * dx + P(d.grady) * dy);
-        P part, sn, cs;
-        if (d.harm == 1) {
-          phase1();
-          sn = sn1;
-          cs = cs1;
-        } else if constexpr (sizeof(P) == 8) {
-          sincos(static_cast<double>(d.harm) * th, &sn, &cs);
-        } else {
-          fsincos(static_cast<double>(d.harm) * th, sn, cs);
-        }
-        if constexpr (sizeof(P) == 8)
-          part = sqrt(faw2) * scl / g;
-        else
-          part = psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g);
-        const P cre = sn * part;
-        const P cim = cs * part;
-        const P w[4] = {wx * wy, (P(1) - wx) * wy, wx * (P(1) - wy), (P(1) - wx) * (P(1) - wy)};
-        deposit_one<P>(cre, cim, w, ix, iy, wx0, wy0, ww, wh, ng, win, src, s_fb);
-      }
-    }
-  }
-
   if constexpr (NH > 0) {
     const double dg = gshift(g, g0);
     ps.e[0] += dg;
