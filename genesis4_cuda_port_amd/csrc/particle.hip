@@ -85,7 +85,9 @@ __device__ __forceinline__ void fsincos(double a, float& s, float& c) {
 }
 
 // ---- transverse ---------------------------------------------------------
-// FP64: advance_focusing_plane (TrackBeamCUDA.cpp:27-59) verbatim.
+// advance_focusing_plane (TrackBeamCUDA.cpp:27-59) in the reference's
+// trigonometric form; FP64 takes it where the power series below does not
+// apply (a strong quadrupole over a long step).
 template <class P>
 __device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& coord, P& mom,
                                            P gz) {
@@ -169,6 +171,47 @@ __device__ __forceinline__ void plane_step(int mode, float str, float off, float
   mom = a3 * xt * gz + a1 * mom;
 }
 
+// FP64 with `poly`: the power series of the MIXED overload below taken to
+// w^8 (for |w| <= 1/2 the remainders are below 1e-18, under FP64
+// rounding), with one reciprocal of gammaz for both planes; otherwise the
+// reference's trigonometric form.
+__device__ __forceinline__ void plane_step(int mode, double str, double off, double delz,
+                                           double& coord, double& mom, double gz, double igz,
+                                           bool poly) {
+  if (mode == 0) {
+    coord += mom * delz * igz;
+    return;
+  }
+  if (!poly) {
+    plane_step<double>(mode, str, off, delz, coord, mom, gz);
+    return;
+  }
+  const double k = str * igz;
+  const double w = k * delz * delz;
+  double c = 1.0 / 20922789888000.0;  // cos(sqrt(w)): (-w)^n / (2n)!, n <= 8
+  c = fma(c, w, -1.0 / 87178291200.0);
+  c = fma(c, w, 1.0 / 479001600.0);
+  c = fma(c, w, -1.0 / 3628800.0);
+  c = fma(c, w, 1.0 / 40320.0);
+  c = fma(c, w, -1.0 / 720.0);
+  c = fma(c, w, 1.0 / 24.0);
+  c = fma(c, w, -0.5);
+  c = fma(c, w, 1.0);
+  double sn = 1.0 / 355687428096000.0;  // sin(sqrt(w))/sqrt(w): (-w)^n / (2n+1)!
+  sn = fma(sn, w, -1.0 / 1307674368000.0);
+  sn = fma(sn, w, 1.0 / 6227020800.0);
+  sn = fma(sn, w, -1.0 / 39916800.0);
+  sn = fma(sn, w, 1.0 / 362880.0);
+  sn = fma(sn, w, -1.0 / 5040.0);
+  sn = fma(sn, w, 1.0 / 120.0);
+  sn = fma(sn, w, -1.0 / 6.0);
+  sn = fma(sn, w, 1.0);
+  const double a2 = delz * sn;  // sin(omega)/foc
+  const double xt = coord - off;
+  coord = fma(c, xt, a2 * mom * igz) + off;
+  mom = fma(-k * a2 * gz, xt, c * mom);
+}
+
 __device__ __forceinline__ double psqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float psqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 
@@ -201,9 +244,10 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
     py += P(t.cy);
   }
   if constexpr (sizeof(P) == 8) {
-    const P gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
-    plane_step(t.xmode, P(t.xstr), P(t.xoff), P(t.delz), x, px, gz);
-    plane_step(t.ymode, P(t.ystr), P(t.yoff), P(t.delz), y, py, gz);
+    const double gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
+    const double igz = 1.0 / gz;
+    plane_step(t.xmode, t.xstr, t.xoff, t.delz, x, px, gz, igz, t.xpoly != 0);
+    plane_step(t.ymode, t.ystr, t.yoff, t.delz, y, py, gz, igz, t.ypoly != 0);
   } else {
     const float gzz = gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py;
     const float igz = __builtin_amdgcn_rsqf(gzz);  // 1/gammaz
