@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,40 @@ void Context::collect() {
     event_pool.push_back(p.b);
   }
   pending.clear();
+}
+
+void* device_alloc(Context* ctx, size_t bytes) {
+  void* p = nullptr;
+  if (!ctx) {
+    G4_HIP(hipMalloc(&p, bytes));
+    return p;
+  }
+  static std::mutex mu;
+  static std::vector<int> pool_ready;
+  {
+    // keep freed blocks in the pool instead of returning them to the driver
+    std::lock_guard<std::mutex> lk(mu);
+    if (pool_ready.size() <= static_cast<size_t>(ctx->device))
+      pool_ready.resize(static_cast<size_t>(ctx->device) + 1, 0);
+    if (!pool_ready[static_cast<size_t>(ctx->device)]) {
+      hipMemPool_t pool;
+      G4_HIP(hipDeviceGetDefaultMemPool(&pool, ctx->device));
+      uint64_t keep = ~0ull;
+      G4_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+      pool_ready[static_cast<size_t>(ctx->device)] = 1;
+    }
+  }
+  G4_HIP(hipMallocAsync(&p, bytes, ctx->stream));
+  G4_HIP(hipStreamSynchronize(ctx->stream));
+  return p;
+}
+
+void device_free(Context* ctx, void* p) {
+  if (!p) return;
+  if (ctx)
+    (void)hipFreeAsync(p, ctx->stream);
+  else
+    (void)hipFree(p);
 }
 
 const void* Context::param_push(const void* data, size_t bytes) {

@@ -10,8 +10,8 @@
 // one field-sum map.  Per-run step parameters (the taper changes aw, and with
 // it the focusing, the coupling fc(aw) and the phase-equation constants)
 // reach the particle pass through a per-slice parameter table, built on the
-// host from the same make_trans / make_push / make_dep as every other path
-// and copied through a pinned ring of tables.
+// host from the same make_trans / make_push / make_dep as every other path,
+// for every step at creation, and kept on the device ([step][run]).
 //
 // Each group has its own context (its own stream), so the groups' launches
 // overlap on the device; g4h_batch_advance joins them into the caller's
@@ -37,8 +37,6 @@ using namespace g4;
 
 namespace {
 
-constexpr int kRing = 8;  // parameter tables in flight per group
-
 void ok(g4h_status s) {
   if (s != G4H_OK) throw Error(s, g4h_last_error());
 }
@@ -50,11 +48,11 @@ struct Group {
   std::vector<int> runs;
   g4h_beam* beam = nullptr;
   g4h_field* field = nullptr;
-  DevBuf<ParticleParams> dtab;       // kRing tables of runs.size() entries
-  ParticleParams* htab = nullptr;    // pinned host copy of the ring
-  hipEvent_t used[kRing] = {};       // last launch that read each ring slot
+  // parameter tables of every step ([step][run]), built once at creation
+  DevBuf<ParticleParams> dtab;
+  std::vector<ParticleParams> head;  // host copy of each step's first entry
+  std::vector<uint8_t> src_on;       // per step: deposition on
   hipEvent_t tail = nullptr;         // end of the group's work so far
-  bool pending[kRing] = {};
   DevBuf<double> beam_hist, field_hist;
 };
 
@@ -71,16 +69,12 @@ struct g4h_batch {
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // run -> (group, slice in group)
   int done = 0, nrec = 0, cap = 0;
-  int slot = 0;
 
   ~g4h_batch() {
     for (Group& g : groups) {
       if (g.field) g4h_field_destroy(g.field);
       if (g.beam) g4h_beam_destroy(g.beam);
-      for (hipEvent_t e : g.used)
-        if (e) (void)hipEventDestroy(e);
       if (g.tail) (void)hipEventDestroy(g.tail);
-      if (g.htab) (void)hipHostFree(g.htab);
       g.dtab.release();
       g.beam_hist.release();
       g.field_hist.release();
@@ -107,49 +101,67 @@ struct g4h_batch {
     nrec = 1;
   }
 
-  void step(int istep) {
-    const g4h_step& st0 = steps[static_cast<size_t>(istep)];
-    const double delz = st0.delz;
-    const bool out = st0.outstep && nrec < cap;
-    const int sl = slot;
-    slot = (slot + 1) % kRing;
+  // Every step's parameter table of every group, built on the host with the
+  // same make_trans / make_push / make_dep as every other path and uploaded
+  // once, so that a step enqueues launches only.
+  void build_tables() {
+    const int ns = static_cast<int>(steps.size());
+    std::vector<int> rec(static_cast<size_t>(ns), -1);
+    int r = 1;
+    for (int k = 0; k < ns; ++k)
+      if (steps[static_cast<size_t>(k)].outstep && r < cap) rec[static_cast<size_t>(k)] = r++;
     for (Group& g : groups) {
       const int n = static_cast<int>(g.runs.size());
-      if (g.pending[sl]) G4_HIP(hipEventSynchronize(g.used[sl]));
-      field_prepare_solver(g.field, delz);
-      ParticleParams* tab = g.htab + static_cast<size_t>(sl) * n;
-      bool src_on = false;
-      for (int i = 0; i < n; ++i) {
-        g4h_step st = st0;
-        const auto& a = aw[static_cast<size_t>(g.runs[static_cast<size_t>(i)])];
-        if (!a.empty()) st.aw = a[static_cast<size_t>(istep)];
-        ParticleParams p{};
-        p.gamma0 = g.gamma0;
-        p.stages = ST_T1 | ST_PUSH | ST_T2;
-        p.t1 = make_trans(st, g.gamma0, delz * 0.5, false);
-        p.t2 = make_trans(st, g.gamma0, delz * 0.5, true);
-        Field* fl[1] = {g.field};
-        make_push(p.push, st, g.gamma0, delz, fl, 1, nullptr);
-        p.r56 = make_r56(st, g.gamma0, lambda0);
-        if (p.r56 != 0.0) p.stages |= ST_R56;
-        const bool on = make_dep(p.dep, g.field, g.beam, st, delz) && g.beam->np > 0;
-        require(i == 0 || on == src_on, "source switched on for part of a batch group");
-        src_on = on;
-        if (on) p.stages |= ST_DEP;
-        if (out) {
-          p.stages |= ST_DIAG;
-          p.diag.raw = g.beam_hist.get() + static_cast<size_t>(nrec) * beam_rec(g);
-          p.diag.nharm = nharm;
+      std::vector<ParticleParams> tab(static_cast<size_t>(ns) * n);
+      g.head.resize(static_cast<size_t>(ns));
+      g.src_on.assign(static_cast<size_t>(ns), 0);
+      for (int k = 0; k < ns; ++k) {
+        const g4h_step& st0 = steps[static_cast<size_t>(k)];
+        const double delz = st0.delz;
+        bool src_on = false;
+        for (int i = 0; i < n; ++i) {
+          g4h_step st = st0;
+          const auto& a = aw[static_cast<size_t>(g.runs[static_cast<size_t>(i)])];
+          if (!a.empty()) st.aw = a[static_cast<size_t>(k)];
+          ParticleParams p{};
+          p.gamma0 = g.gamma0;
+          p.stages = ST_T1 | ST_PUSH | ST_T2;
+          p.t1 = make_trans(st, g.gamma0, delz * 0.5, false);
+          p.t2 = make_trans(st, g.gamma0, delz * 0.5, true);
+          Field* fl[1] = {g.field};
+          make_push(p.push, st, g.gamma0, delz, fl, 1, nullptr);
+          p.r56 = make_r56(st, g.gamma0, lambda0);
+          if (p.r56 != 0.0) p.stages |= ST_R56;
+          const bool on = make_dep(p.dep, g.field, g.beam, st, delz) && g.beam->np > 0;
+          require(i == 0 || on == src_on, "source switched on for part of a batch group");
+          src_on = on;
+          if (on) p.stages |= ST_DEP;
+          const int rk = rec[static_cast<size_t>(k)];
+          if (rk >= 0) {
+            p.stages |= ST_DIAG;
+            p.diag.raw = g.beam_hist.get() + static_cast<size_t>(rk) * beam_rec(g);
+            p.diag.nharm = nharm;
+          }
+          tab[static_cast<size_t>(k) * n + i] = p;
         }
-        tab[i] = p;
+        g.head[static_cast<size_t>(k)] = tab[static_cast<size_t>(k) * n];
+        g.src_on[static_cast<size_t>(k)] = src_on ? 1 : 0;
       }
-      ParticleParams* dtab = g.dtab.get() + static_cast<size_t>(sl) * n;
-      G4_HIP(hipMemcpyAsync(dtab, tab, sizeof(ParticleParams) * n, hipMemcpyHostToDevice,
-                            g.gctx->stream));
-      launch_particle(g.gctx, g.beam, tab[0], dtab);
-      G4_HIP(hipEventRecord(g.used[sl], g.gctx->stream));
-      g.pending[sl] = true;
-      field_solve(g.field, src_on, out);
+      g.dtab.alloc(g.gctx, tab.size());
+      G4_HIP(hipMemcpy(g.dtab.get(), tab.data(), sizeof(ParticleParams) * tab.size(),
+                       hipMemcpyHostToDevice));
+    }
+  }
+
+  void step(int istep) {
+    const g4h_step& st0 = steps[static_cast<size_t>(istep)];
+    const bool out = st0.outstep && nrec < cap;
+    for (Group& g : groups) {
+      const size_t n = g.runs.size();
+      field_prepare_solver(g.field, st0.delz);
+      launch_particle(g.gctx, g.beam, g.head[static_cast<size_t>(istep)],
+                      g.dtab.get() + static_cast<size_t>(istep) * n);
+      field_solve(g.field, g.src_on[static_cast<size_t>(istep)] != 0, out);
       if (out) field_map_stats(g.field, g.field_hist.get() + static_cast<size_t>(nrec) * field_rec(g));
     }
     if (out) ++nrec;
@@ -250,15 +262,11 @@ g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solve
     ok(g4h_field_create(g.gctx, precision, n, g.G, g.gridmax, lambda0, lambda0, 1, &g.field));
     ok(g4h_field_upload(g.field, re.data(), im.data(), 0));
     ok(g4h_field_init_solver(g.field, solver, 0, 1.0, 1.0, 1.0, 0));
-    g.dtab.alloc(g.gctx, static_cast<size_t>(kRing) * n);
-    void* h = nullptr;
-    G4_HIP(hipHostMalloc(&h, sizeof(ParticleParams) * kRing * n, hipHostMallocDefault));
-    g.htab = static_cast<ParticleParams*>(h);
-    for (hipEvent_t& e : g.used) G4_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     G4_HIP(hipEventCreateWithFlags(&g.tail, hipEventDisableTiming));
     g.beam_hist.alloc(g.gctx, static_cast<size_t>(b->cap) * b->beam_rec(g));
     g.field_hist.alloc(g.gctx, static_cast<size_t>(b->cap) * b->field_rec(g));
   }
+  b->build_tables();
   b->record_initial();
   *out = b.release();
   context_child_added(ctx);

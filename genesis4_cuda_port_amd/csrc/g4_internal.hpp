@@ -216,13 +216,20 @@ class TuneScope {
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
+// Device memory of a context comes from the device's stream-ordered pool:
+// hipFreeAsync returns a buffer after the work already enqueued on the
+// context's stream, without the device-wide synchronisation of hipFree (a
+// request loop releasing one lane's memory would otherwise stall every other
+// lane).  An allocation waits for its stream, so the buffer is free of any
+// earlier work before other streams or blocking copies touch it.
+void* device_alloc(Context* ctx, size_t bytes);
+void device_free(Context* ctx, void* p);
+
 template <class T>
 void DevBuf<T>::alloc(Context* ctx, size_t n) {
   release();
   if (n == 0) return;
-  void* p = nullptr;
-  G4_HIP(hipMalloc(&p, n * sizeof(T)));
-  p_ = static_cast<T*>(p);
+  p_ = static_cast<T*>(device_alloc(ctx, n * sizeof(T)));
   n_ = n;
   ctx_ = ctx;
   if (ctx_) ctx_->bytes += n * sizeof(T);
@@ -231,7 +238,7 @@ void DevBuf<T>::alloc(Context* ctx, size_t n) {
 template <class T>
 void DevBuf<T>::release() {
   if (p_) {
-    (void)hipFree(p_);
+    device_free(ctx_, p_);
     if (ctx_) ctx_->bytes -= n_ * sizeof(T);
   }
   p_ = nullptr;

@@ -8,13 +8,23 @@
 // of guessing: the first kReps * nvar calls of a (family, shape) run the
 // variants in turn between two events; once every trial's end event has
 // completed (checked with hipEventQuery, so the host never waits) the
-// variant with the smallest best time is fixed for that shape.
+// variant with the smallest best time is fixed for that shape, for this
+// context and for contexts created later in the process on the same device.
 #include <algorithm>
 #include <limits>
+#include <mutex>
 
 #include "kernels.hpp"
 
 namespace g4 {
+
+namespace {
+// Decisions shared by all contexts of the process, keyed by device, family
+// and shape: a new context (a batch group, a request lane) starts from what
+// earlier contexts measured instead of trying the variants again.
+std::mutex g_mu;
+std::map<std::vector<int>, int> g_decided;
+}  // namespace
 
 int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipEvent_t* a,
                        hipEvent_t* b) {
@@ -30,6 +40,16 @@ int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipE
   Tuner::Entry& e = tuner.table[key];
   if (e.nvar == 0) e.nvar = nvar;
   if (e.choice >= 0) return e.choice;
+  std::vector<int> gkey{device};
+  gkey.insert(gkey.end(), key.begin(), key.end());
+  if (e.issued == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_decided.find(gkey);
+    if (it != g_decided.end()) {
+      e.choice = it->second;
+      return e.choice;
+    }
+  }
   const int total = Tuner::kReps * e.nvar;
   if (e.issued < total) {
     // next trial: variants in turn, so every variant sees warm and cold caches
@@ -58,6 +78,10 @@ int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipE
   }
   e.trials.clear();
   e.choice = static_cast<int>(std::min_element(best.begin(), best.end()) - best.begin());
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_decided[gkey] = e.choice;
+  }
   return e.choice;
 }
 
