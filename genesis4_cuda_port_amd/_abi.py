@@ -136,6 +136,8 @@ PROTOTYPES = {
     "g4h_beam_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp, C.c_double, C.POINTER(_vp)]),
     "g4h_beam_destroy": (None, [_vp]),
     "g4h_beam_upload": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "g4h_beam_generate": (C.c_int, [_vp, C.POINTER(BeamSpec), _dp]),
+    "g4h_field_fill_planes": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "g4h_beam_download": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "g4h_beam_set_current": (C.c_int, [_vp, _dp]),
     "g4h_beam_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),

@@ -144,6 +144,12 @@ class Beam:
         check(lib().g4h_beam_download(self.h, *map(_dp, arrs)))
         return tuple(arrs)
 
+    def generate(self, **kw):
+        """Quiet-start beam generated on the device (g4h_beam_generate);
+        keywords of beam_spec."""
+        spec, cur = beam_spec(**kw)
+        check(lib().g4h_beam_generate(self.h, C.byref(spec), _dp(cur)))
+
     def track_transverse(self, step: Step, delz: float, last: bool):
         check(lib().g4h_track_transverse(self.ctx.h, self.h, C.byref(step), delz, int(last)))
 
@@ -199,6 +205,10 @@ class Field:
     def upload(self, re, im, first: int = 0):
         check(lib().g4h_field_upload(self.h, _dp(_f64(re, self.cells)), _dp(_f64(im, self.cells)),
                                      first))
+
+    def fill_planes(self, re, im, first: int = 0):
+        """Every ring plane <- one plane (g4h_field_fill_planes)."""
+        check(lib().g4h_field_fill_planes(self.h, _dp(_f64(re)), _dp(_f64(im)), first))
 
     def download(self):
         re = np.empty(self.cells)
@@ -505,17 +515,25 @@ class BeamArrays:
                             ("x", "y", "px", "py", "gamma", "theta", "offsets", "current")))
 
 
-def load_beam(*, nslice: int, npart: int, nbins: int, gamma0: float, espread_rel: float,
+def beam_spec(*, nslice: int, npart: int, nbins: int, gamma0: float, espread_rel: float,
               emitn: float, beta: float, alpha: float = 0.0, slicelength: float, current,
               shotnoise: bool, seed: int, chirp_rel: float = 0.0, emitn_y: float | None = None,
-              beta_y: float | None = None, alpha_y: float | None = None) -> BeamArrays:
-    """Quiet-start beam (g4h_load_beam); the y plane defaults to the x plane."""
+              beta_y: float | None = None, alpha_y: float | None = None):
+    """(g4h_beam_spec, per-slice currents) of a quiet-start beam; the y plane
+    defaults to the x plane."""
     cur = _f64(np.broadcast_to(np.asarray(current, dtype=np.float64), (nslice,)).copy(), nslice)
     ey = emitn if emitn_y is None else emitn_y
     by = beta if beta_y is None else beta_y
     ay = alpha if alpha_y is None else alpha_y
     spec = BeamSpec(gamma0, espread_rel, chirp_rel, emitn, ey, beta, by, alpha, ay,
                     0.0, 0.0, 0.0, 0.0, slicelength, nslice, npart, nbins, int(shotnoise), seed)
+    return spec, cur
+
+
+def load_beam(**kw) -> BeamArrays:
+    """Quiet-start beam on the host (g4h_load_beam); keywords of beam_spec."""
+    spec, cur = beam_spec(**kw)
+    nslice, npart = spec.nslice, spec.npart
     n = nslice * npart
     arrs = [np.empty(n) for _ in range(6)]
     check(lib().g4h_load_beam(C.byref(spec), _dp(cur), *map(_dp, arrs)))

@@ -27,6 +27,7 @@ SOURCES = [
     "field_fft.hip",
     "field_adi.hip",
     "field_misc.hip",
+    "loader.hip",
     "abi.cpp",
     "run.cpp",
     "batch.cpp",

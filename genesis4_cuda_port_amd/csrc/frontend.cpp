@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/g4hip.h"
+#include "loader.hpp"
 
 namespace g4 {
 void set_last_error(const std::string& m);
@@ -65,25 +66,44 @@ void sobol_point(int dims, uint64_t n, const std::vector<uint32_t>& v, uint32_t*
   }
 }
 
-// splitmix64 (Steele, Lea, Flood 2014) for independent per-slice seeds.
-uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
-double uniform01(std::mt19937_64& r) {
-  return (static_cast<double>(r() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-// Box-Muller normal deviate.
-double normal(std::mt19937_64& r) {
-  const double u1 = uniform01(r), u2 = uniform01(r);
-  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
-}
-
 }  // namespace
+
+namespace g4 {
+// Loader parameters of a beam specification (shared with g4h_beam_generate).
+bool loader_spec(const g4h_beam_spec* sp, loader::Spec& out) {
+  if (!sp || sp->nslice < 1 || sp->npart < 1 || sp->nbins < 1 || sp->nbins > 64 ||
+      (sp->npart % sp->nbins) != 0 || sp->gamma0 <= 1.0 || sp->betax <= 0 || sp->betay <= 0)
+    return false;
+  std::vector<uint32_t> v;
+  sobol_directions(loader::kDims, v);
+  for (int d = 0; d < loader::kDims; ++d)
+    for (int i = 0; i < 32; ++i) out.v[d][i] = v[static_cast<size_t>(d) * 32 + i];
+  // digital shift (XOR with seeded random words): a randomised Sobol net
+  std::mt19937_64 rs(sp->seed);
+  for (int d = 0; d < loader::kDims; ++d) out.shift[d] = static_cast<uint32_t>(rs() >> 32);
+  const double ex = sp->emitn_x / sp->gamma0, ey = sp->emitn_y / sp->gamma0;
+  out.sx = std::sqrt(ex * sp->betax);
+  out.sxp = std::sqrt(ex / sp->betax);
+  out.sy = std::sqrt(ey * sp->betay);
+  out.syp = std::sqrt(ey / sp->betay);
+  out.alphax = sp->alphax;
+  out.alphay = sp->alphay;
+  out.gamma0 = sp->gamma0;
+  out.espread_rel = sp->espread_rel;
+  out.chirp_rel = sp->chirp_rel;
+  out.xcen = sp->xcen;
+  out.ycen = sp->ycen;
+  out.pxcen = sp->pxcen;
+  out.pycen = sp->pycen;
+  out.slicelength = sp->slicelength;
+  out.seed = sp->seed;
+  out.nslice = sp->nslice;
+  out.npart = sp->npart;
+  out.nbins = sp->nbins;
+  out.shotnoise = sp->shotnoise;
+  return true;
+}
+}  // namespace g4
 
 extern "C" {
 
@@ -115,103 +135,46 @@ g4h_status g4h_sobol(int dims, int64_t index0, int64_t n, double* out) {
   return G4H_OK;
 }
 
-// Inverse standard normal CDF: Halley iteration on Phi(x) - p with
-// Phi(x) = erfc(-x/sqrt2)/2, started from the Tukey-lambda approximation
-// x0 = 4.91 (p^0.14 - (1-p)^0.14).  The upper half uses Phi^-1(p) =
-// -Phi^-1(1-p) (1-p is exact there) so both tails are solved where erfc is
-// accurate.
+// Inverse standard normal CDF (the loader's Halley iteration, loader.hpp).
 double g4h_inverse_normal(double p) {
   if (!(p > 0.0 && p < 1.0)) {
     if (p == 0.0) return -HUGE_VAL;
     if (p == 1.0) return HUGE_VAL;
     return NAN;
   }
-  if (p > 0.5) return -g4h_inverse_normal(1.0 - p);
-  double x = 4.91 * (std::pow(p, 0.14) - std::pow(1.0 - p, 0.14));
-  for (int it = 0; it < 60; ++it) {
-    const double phi = 0.5 * std::erfc(-x / std::sqrt(2.0));
-    const double pdf = std::exp(-0.5 * x * x) / std::sqrt(2.0 * kPi);
-    const double f = phi - p;
-    if (pdf <= 0.0) break;
-    const double r = f / pdf;
-    const double dx = r / (1.0 + 0.5 * x * r);
-    x -= dx;
-    if (std::fabs(dx) <= 1e-16 * std::fmax(1.0, std::fabs(x))) break;
-  }
-  return x;
+  return g4::loader::inverse_normal(p);
 }
+
 
 g4h_status g4h_load_beam(const g4h_beam_spec* sp, const double* current, double* x, double* y,
                          double* px, double* py, double* gamma, double* theta) {
-  if (!sp || !x || !y || !px || !py || !gamma || !theta || sp->nslice < 1 || sp->npart < 1 ||
-      sp->nbins < 1 || (sp->npart % sp->nbins) != 0 || sp->gamma0 <= 1.0 || sp->betax <= 0 ||
-      sp->betay <= 0 || (sp->shotnoise && !current)) {
+  g4::loader::Spec ls;
+  if (!x || !y || !px || !py || !gamma || !theta || (sp && sp->shotnoise && !current) ||
+      !g4::loader_spec(sp, ls)) {
     g4::set_last_error("g4h_load_beam: invalid specification");
     return G4H_ERR_INVALID_ARGUMENT;
   }
   const int nb = sp->npart / sp->nbins;
-  std::vector<uint32_t> v;
-  sobol_directions(6, v);
-  // digital shift (XOR with seeded random words): a randomised Sobol net
-  std::mt19937_64 rs(sp->seed);
-  uint32_t shift[6];
-  for (int d = 0; d < 6; ++d) shift[d] = static_cast<uint32_t>(rs() >> 32);
-  const double ex = sp->emitn_x / sp->gamma0, ey = sp->emitn_y / sp->gamma0;
-  const double sx = std::sqrt(ex * sp->betax), sxp = std::sqrt(ex / sp->betax);
-  const double sy = std::sqrt(ey * sp->betay), syp = std::sqrt(ey / sp->betay);
-  const double twopi = 2.0 * kPi;
-  // Slices are independent (Sobol index and RNG stream are per slice), so
-  // they are loaded by a pool of host threads; the result does not depend on
-  // the thread count.
+  // Slices are independent, so they are loaded by a pool of host threads;
+  // the result does not depend on the thread count.
   auto load_slices = [&](int s0, int s1) {
-  std::vector<double> th0(sp->nbins);
-  for (int is = s0; is < s1; ++is) {
-    const double chirp =
-        sp->nslice > 1 ? sp->chirp_rel * sp->gamma0 * (static_cast<double>(is) / (sp->nslice - 1) - 0.5) : 0.0;
-    std::mt19937_64 rn(splitmix64(sp->seed ^ (0x5851F42D4C957F2Dull * static_cast<uint64_t>(is + 1))));
-    const double ne = sp->shotnoise ? current[is] * sp->slicelength / G4H_CE / nb : 0.0;
-    for (int j = 0; j < nb; ++j) {
-      uint32_t pt[6];
-      sobol_point(6, 1 + static_cast<uint64_t>(is) * nb + j, v, pt);
-      double u[6];
-      for (int d = 0; d < 6; ++d)
-        u[d] = (static_cast<double>(pt[d] ^ shift[d]) + 0.5) * (1.0 / 4294967296.0);
-      const double gx = g4h_inverse_normal(u[0]);
-      const double gpx = g4h_inverse_normal(u[1]);
-      const double gy = g4h_inverse_normal(u[2]);
-      const double gpy = g4h_inverse_normal(u[3]);
-      const double gg = g4h_inverse_normal(u[4]);
-      const double xv = sp->xcen + sx * gx;
-      const double pxv = sp->pxcen + sp->gamma0 * sxp * (gpx - sp->alphax * gx);
-      const double yv = sp->ycen + sy * gy;
-      const double pyv = sp->pycen + sp->gamma0 * syp * (gpy - sp->alphay * gy);
-      const double gv = sp->gamma0 * (1.0 + sp->espread_rel * gg) + chirp;
-      const double base = u[5] * twopi / sp->nbins;
-      for (int k = 0; k < sp->nbins; ++k) th0[k] = base + twopi * k / sp->nbins;
-      // Fawley (PRST-AB 5, 070701, 2002): phases perturbed by
-      // sum_h a_h cos(h th) + b_h sin(h th) with a_h, b_h ~ N(0, 2/(h^2 Ne))
-      // (half that variance for h = nbins/2), so each beamlet's bunching has
-      // the statistics of Ne random electrons.
-      const int64_t i0 = static_cast<int64_t>(sp->npart) * is + static_cast<int64_t>(j) * sp->nbins;
-      for (int k = 0; k < sp->nbins; ++k) {
-        x[i0 + k] = xv;
-        px[i0 + k] = pxv;
-        y[i0 + k] = yv;
-        py[i0 + k] = pyv;
-        gamma[i0 + k] = gv;
-        theta[i0 + k] = th0[k];
-      }
-      if (sp->shotnoise && ne > 0.0) {
-        for (int h = 1; h <= sp->nbins / 2; ++h) {
-          const bool nyq = (sp->nbins % 2 == 0) && (2 * h == sp->nbins);
-          const double sig = std::sqrt((nyq ? 1.0 : 2.0) / (static_cast<double>(h) * h * ne));
-          const double a = sig * normal(rn), b = sig * normal(rn);
-          for (int k = 0; k < sp->nbins; ++k)
-            theta[i0 + k] += a * std::cos(h * th0[k]) + b * std::sin(h * th0[k]);
+    double th[64];
+    for (int is = s0; is < s1; ++is) {
+      const double cur = sp->shotnoise ? current[is] : 0.0;
+      for (int j = 0; j < nb; ++j) {
+        double xv, pxv, yv, pyv, gv;
+        g4::loader::beamlet(ls, is, j, cur, G4H_CE, xv, pxv, yv, pyv, gv, th);
+        const int64_t i0 = static_cast<int64_t>(sp->npart) * is + static_cast<int64_t>(j) * sp->nbins;
+        for (int k = 0; k < sp->nbins; ++k) {
+          x[i0 + k] = xv;
+          px[i0 + k] = pxv;
+          y[i0 + k] = yv;
+          py[i0 + k] = pyv;
+          gamma[i0 + k] = gv;
+          theta[i0 + k] = th[k];
         }
       }
     }
-  }
   };
   int nthr = static_cast<int>(std::thread::hardware_concurrency());
   if (const char* e = std::getenv("OMP_NUM_THREADS")) nthr = std::atoi(e);

@@ -9,13 +9,12 @@
 // different lanes overlaps on the device.  Steady-state requests that share
 // a lattice, precision and solver and are queued together run as one batch
 // (g4h_batch: one launch per step for all of them); a time-dependent request
-// is one fused run (g4h_run).  The host state of a time-dependent request
-// (its quiet-start beam and seed field, up to a few GB and ~1 s of host
-// work) is built by preparation threads ahead of admission, so the
-// scheduler thread only creates, uploads and enqueues; a request whose state
-// is not ready yet does not hold back the requests queued behind it.
-// Latency is measured from submission to the moment its records are on the
-// host.
+// is one fused run (g4h_run) whose beam is generated and whose seed field
+// is filled on the device (g4h_beam_generate, g4h_field_fill_planes): the
+// scheduler thread never builds host arrays of a request's size, which
+// took ~1 s of host time for the largest requests and held every other lane
+// back meanwhile.  Latency is measured from submission to the moment its
+// records are on the host.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -53,18 +52,7 @@ struct Req {
   std::string error;
   int nrec = 0;
   std::vector<double> braw, fraw;
-  // host state of a time-dependent request (beam, slice currents, seed
-  // field), built by a preparation thread ahead of admission; prep is 0
-  // waiting, 1 building, 2 ready, 3 failed (guarded by the server mutex)
-  std::vector<double> hx, hy, hpx, hpy, hg, hth, cur, hre, him;
-  int prep = 0;
-  bool ahead = false;  // counted in the server's prep_ahead
 };
-
-// Time-dependent requests whose host state may be built ahead of admission
-// (each holds up to a few GB of host memory until it is uploaded).
-constexpr int kPrepAhead = 4;
-constexpr int kPrepThreads = 2;
 
 // HBM estimate of a request's resident state (beam, field ring, source
 // planes, FP64 ADI intermediate, diagnostics history).
@@ -102,10 +90,6 @@ struct g4h_server {
   uint64_t budget = 0;
   std::vector<Lane> lanes;
   std::thread worker;
-  std::vector<std::thread> preps;
-  std::deque<std::shared_ptr<Req>> prep_q;
-  std::condition_variable cv_prep;
-  int prep_ahead = 0;  // requests being built or built and not yet admitted
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Req>> queue;
@@ -119,7 +103,6 @@ struct g4h_server {
   std::string fatal;
 
   void loop();
-  void prep_loop();
   bool admit(Lane& lane);
   void finish(Lane& lane);
   void fail_lane(Lane& lane, const std::string& msg, int status);
@@ -139,79 +122,7 @@ bool same_batch(const Req& a, const Req& b) {
          std::memcmp(a.steps.data(), b.steps.data(), a.steps.size() * sizeof(g4h_step)) == 0;
 }
 
-// Host state of a time-dependent request: quiet-start beam with shot noise
-// and the Gaussian seed field (zero without seed power).
-void prepare(Req& r) {
-  const g4h_request& q = r.q;
-  g4h_beam_spec s{};
-  s.gamma0 = q.gamma0;
-  s.espread_rel = q.espread_rel;
-  s.chirp_rel = q.chirp_rel;
-  s.emitn_x = s.emitn_y = q.emitn;
-  s.betax = s.betay = q.beta;
-  s.slicelength = q.lambda0 * q.sample;
-  s.nslice = q.nslice;
-  s.npart = q.npart;
-  s.nbins = q.nbins;
-  s.shotnoise = q.shotnoise;
-  s.seed = q.seed;
-  const size_t np = static_cast<size_t>(q.nslice) * q.npart;
-  r.hx.resize(np);
-  r.hy.resize(np);
-  r.hpx.resize(np);
-  r.hpy.resize(np);
-  r.hg.resize(np);
-  r.hth.resize(np);
-  r.cur.assign(static_cast<size_t>(q.nslice), q.current);
-  ok(g4h_load_beam(&s, r.cur.data(), r.hx.data(), r.hy.data(), r.hpx.data(), r.hpy.data(),
-                   r.hg.data(), r.hth.data()));
-  r.hre.assign(static_cast<size_t>(q.nslice) * q.ngrid * q.ngrid, 0.0);
-  r.him.assign(r.hre.size(), 0.0);
-  if (q.seed_power > 0)
-    ok(g4h_seed_gaussian(q.nslice, q.ngrid, q.gridmax, q.lambda0, q.seed_power, q.seed_w0,
-                         r.hre.data(), r.him.data()));
-}
-
-void drop_host_state(Req& r) {
-  for (auto* v : {&r.hx, &r.hy, &r.hpx, &r.hpy, &r.hg, &r.hth, &r.cur, &r.hre, &r.him}) {
-    v->clear();
-    v->shrink_to_fit();
-  }
-}
-
 }  // namespace
-
-// Builds the host state of queued time-dependent requests in submission
-// order, at most kPrepAhead ahead of admission.
-void g4h_server::prep_loop() {
-  for (;;) {
-    std::shared_ptr<Req> r;
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_prep.wait(lk, [&] {
-        return (stop && prep_q.empty()) || (!prep_q.empty() && prep_ahead < kPrepAhead);
-      });
-      if (prep_q.empty()) return;
-      r = prep_q.front();
-      prep_q.pop_front();
-      r->prep = 1;
-      r->ahead = true;
-      ++prep_ahead;
-    }
-    std::string err;
-    try {
-      prepare(*r);
-    } catch (const std::exception& e) {
-      err = e.what();
-    }
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      r->prep = err.empty() ? 2 : 3;
-      if (!err.empty()) r->error = err;
-    }
-    cv_work.notify_all();
-  }
-}
 
 void g4h_server::clear_lane(Lane& lane) {
   if (lane.batch) g4h_batch_destroy(lane.batch);
@@ -230,8 +141,6 @@ void g4h_server::fail_lane(Lane& lane, const std::string& msg, int status) {
   {
     std::lock_guard<std::mutex> lk(mu);
     for (auto& r : lane.reqs) {
-      if (r->ahead) --prep_ahead;
-      r->ahead = false;
       r->status = status;
       r->error = msg;
       r->t_end = now_s();
@@ -239,55 +148,32 @@ void g4h_server::fail_lane(Lane& lane, const std::string& msg, int status) {
     in_use -= lane.bytes;
     lane.bytes = 0;
   }
-  for (auto& r : lane.reqs) drop_host_state(*r);
   clear_lane(lane);
-  cv_prep.notify_all();
   cv_done.notify_all();
 }
 
-// Take the first queued request that can start (a steady-state one, or a
-// time-dependent one whose host state is built) and, for a steady-state
-// request, the queued requests that can share its batch, if they fit the
-// budget; build and enqueue them on `lane`.  Returns false when nothing was
-// admitted.
+// Take the head of the queue (and, for a steady-state head, the queued
+// requests that can share its batch) if it fits the budget; build and
+// enqueue it on `lane`.  Returns false when nothing was admitted.
 bool g4h_server::admit(Lane& lane) {
   std::vector<std::shared_ptr<Req>> take;
   uint64_t bytes = 0;
   {
     std::lock_guard<std::mutex> lk(mu);
     if (queue.empty()) return false;
-    // the first queued request that can start: steady-state ones at once,
-    // time-dependent ones once their host state is built
-    auto pick = queue.end();
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
-      Req& r = **it;
-      if (r.bytes > budget || (r.q.kind == 1 && r.prep == 3)) {  // can never run
-        auto dead = *it;
-        queue.erase(it);
-        if (dead->q.kind == 1 && dead->prep == 3) {
-          dead->status = G4H_ERR_DEVICE;
-          if (dead->ahead) --prep_ahead;
-          dead->ahead = false;
-          drop_host_state(*dead);
-          cv_prep.notify_all();
-        } else {
-          dead->status = G4H_ERR_OUT_OF_MEMORY;
-          dead->error = "request exceeds the server's HBM budget";
-        }
-        dead->t_end = now_s();
-        cv_done.notify_all();
-        return true;
-      }
-      if (r.q.kind == 1 && r.prep != 2) continue;
-      pick = it;
-      break;
+    auto head = queue.front();
+    if (head->bytes > budget) {  // can never fit
+      queue.pop_front();
+      head->status = G4H_ERR_OUT_OF_MEMORY;
+      head->error = "request exceeds the server's HBM budget";
+      head->t_end = now_s();
+      cv_done.notify_all();
+      return true;
     }
-    if (pick == queue.end()) return false;
-    auto head = *pick;
     if (in_use + head->bytes > budget) return false;
     take.push_back(head);
     bytes = head->bytes;
-    queue.erase(pick);
+    queue.pop_front();
     if (head->q.kind == 0) {
       for (auto it = queue.begin(); it != queue.end();) {
         if (same_batch(*head, **it) && in_use + bytes + (*it)->bytes <= budget) {
@@ -335,23 +221,33 @@ bool g4h_server::admit(Lane& lane) {
                           runs.data(), h.steps.data(), nsteps, q.lambda0, q.nharm, &lane.batch));
       ok(g4h_batch_advance(lane.batch, nsteps, nullptr));
     } else {
-      Req& r = *take.front();
+      // beam generated and field seeded on the device: no host arrays of
+      // the request's size
+      g4h_beam_spec s{};
+      s.gamma0 = q.gamma0;
+      s.espread_rel = q.espread_rel;
+      s.chirp_rel = q.chirp_rel;
+      s.emitn_x = s.emitn_y = q.emitn;
+      s.betax = s.betay = q.beta;
+      s.slicelength = q.lambda0 * q.sample;
+      s.nslice = q.nslice;
+      s.npart = q.npart;
+      s.nbins = q.nbins;
+      s.shotnoise = q.shotnoise;
+      s.seed = q.seed;
+      std::vector<double> cur(static_cast<size_t>(q.nslice), q.current);
       std::vector<int32_t> off(static_cast<size_t>(q.nslice) + 1);
       for (int i = 0; i <= q.nslice; ++i) off[static_cast<size_t>(i)] = i * q.npart;
-      ok(g4h_beam_create(lane.ctx, q.precision, q.nslice, off.data(), r.cur.data(), q.gamma0,
+      ok(g4h_beam_create(lane.ctx, q.precision, q.nslice, off.data(), cur.data(), q.gamma0,
                          &lane.beam));
-      ok(g4h_beam_upload(lane.beam, r.hx.data(), r.hy.data(), r.hpx.data(), r.hpy.data(),
-                         r.hg.data(), r.hth.data()));
+      ok(g4h_beam_generate(lane.beam, &s, cur.data()));
+      std::vector<double> re(static_cast<size_t>(q.ngrid) * q.ngrid, 0.0), im(re.size(), 0.0);
+      if (q.seed_power > 0)
+        ok(g4h_seed_gaussian(1, q.ngrid, q.gridmax, q.lambda0, q.seed_power, q.seed_w0,
+                             re.data(), im.data()));
       ok(g4h_field_create(lane.ctx, q.precision, q.nslice, q.ngrid, q.gridmax, q.lambda0,
                           q.lambda0 * q.sample, 1, &lane.field));
-      ok(g4h_field_upload(lane.field, r.hre.data(), r.him.data(), 0));
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        if (r.ahead) --prep_ahead;
-        r.ahead = false;
-      }
-      drop_host_state(r);
-      cv_prep.notify_all();
+      ok(g4h_field_fill_planes(lane.field, re.data(), im.data(), 0));
       ok(g4h_field_init_solver(lane.field, q.solver, 0, 1.0, 1.0, 1.0, 0));
       g4h_run_config cfg{q.lambda0, q.sample, 1, q.periodic, q.nharm, 1};
       ok(g4h_run_create(lane.ctx, lane.beam, lane.field, h.steps.data(), nsteps, &cfg, &lane.run));
@@ -456,7 +352,6 @@ g4h_status g4h_server_create(int device, uint64_t hbm_budget, int nstreams, g4h_
   }
   g4h_server* raw = s.get();
   s->worker = std::thread([raw] { raw->loop(); });
-  for (int i = 0; i < kPrepThreads; ++i) s->preps.emplace_back([raw] { raw->prep_loop(); });
   *out = s.release();
   G4_API_END
 }
@@ -468,10 +363,7 @@ void g4h_server_destroy(g4h_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  s->cv_prep.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  for (auto& t : s->preps)
-    if (t.joinable()) t.join();
   for (Lane& l : s->lanes) {
     s->clear_lane(l);
     if (l.done_ev) (void)hipEventDestroy(l.done_ev);
@@ -504,11 +396,9 @@ g4h_status g4h_server_submit(g4h_server* s, const g4h_request* q, int64_t* id) {
     if (s->first_submit < 0) s->first_submit = r->t_submit;
     s->all[r->id] = r;
     s->queue.push_back(r);
-    if (r->q.kind == 1) s->prep_q.push_back(r);
     *id = r->id;
   }
   s->cv_work.notify_all();
-  s->cv_prep.notify_all();
   G4_API_END
 }
 

@@ -114,6 +114,11 @@ void g4h_field_destroy(g4h_field* field);
  * `first` is the ring offset (Field::first). */
 g4h_status g4h_field_upload(g4h_field* field, const double* re, const double* im,
                             int first);
+/* Every ring plane <- one plane (re, im of ngrid^2 cells), ring offset
+ * `first`: a seed field that is the same in every slice (upstream's
+ * LoadField with a Gaussian mode) without a host array of the whole field.
+ * No counterpart in the reference (its field is built on the host). */
+g4h_status g4h_field_fill_planes(g4h_field* field, const double* re, const double* im, int first);
 g4h_status g4h_field_download(g4h_field* field, double* re, double* im, int* first);
 /* Device-to-device copy of the planes and ring state of `src` into `dst`
  * (same precision, nslice and ngrid): a state snapshot / clone with one
@@ -352,6 +357,13 @@ typedef struct g4h_beam_spec {
 } g4h_beam_spec;
 g4h_status g4h_load_beam(const g4h_beam_spec* spec, const double* current, double* x,
                          double* y, double* px, double* py, double* gamma, double* theta);
+
+/* Generate the quiet-start beam of `spec` (the particles g4h_load_beam
+ * returns, to round-off) directly into the beam's device arrays; current:
+ * per-slice current (needed for shot noise).  Every slice of the beam must
+ * hold spec->npart particles.  Replaces the host loading and upload of
+ * Beam::init_beamSoA (src/Core/Beam.cpp:342-390) for generated beams. */
+g4h_status g4h_beam_generate(g4h_beam* beam, const g4h_beam_spec* spec, const double* current);
 
 /* Gaussian seed field at its waist: u = u0*exp(-r^2/w0^2) normalised to
  * `power` [W] per slice (power = sum|u|^2 (dgrid eev/ks)^2/vacimp), written

@@ -26,24 +26,34 @@ def relerr(a, b):
 
 
 def _alone(ctx, r, steps):
-    beam = api.load_beam(nslice=r["nslice"], npart=r["npart"], nbins=4, gamma0=GAMMA0,
-                         espread_rel=1e-4, emitn=0.4e-6, beta=15.0,
-                         slicelength=LAMBDA0 * r.get("sample", 1.0), current=r["current"],
-                         shotnoise=True, seed=r["seed"])
+    """The request run alone through g4h_run, from the state the server
+    builds: steady-state beams from the host loader (as g4h_batch_create),
+    time-dependent ones generated on the device with the seed plane filled
+    into every slice (as the server does for them)."""
+    kw = dict(nslice=r["nslice"], npart=r["npart"], nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+              emitn=0.4e-6, beta=15.0, slicelength=LAMBDA0 * r.get("sample", 1.0),
+              current=r["current"], shotnoise=True, seed=r["seed"])
     st = api.steps_array(list(steps))
     if r.get("aw") is not None:
         for i in range(len(st)):
             st[i].aw = float(r["aw"][i])
     S, G = r["nslice"], r["ngrid"]
     if r.get("seed_power", 0) > 0:
-        re, im = api.seed_gaussian(S, G, GRIDMAX, LAMBDA0, r["seed_power"], 50e-6)
+        re, im = api.seed_gaussian(1 if r["kind"] == 1 else S, G, GRIDMAX, LAMBDA0,
+                                   r["seed_power"], 50e-6)
     else:
-        re = np.zeros(S * G * G)
+        re = np.zeros((1 if r["kind"] == 1 else S) * G * G)
         im = np.zeros_like(re)
-    b = api.Beam(ctx, api.FP64, beam.offsets, beam.current, GAMMA0)
-    b.upload(beam.x, beam.y, beam.px, beam.py, beam.gamma, beam.theta)
+    off = (np.arange(S + 1) * r["npart"]).astype(np.int32)
+    b = api.Beam(ctx, api.FP64, off, np.full(S, r["current"]), GAMMA0)
     f = api.Field(ctx, api.FP64, S, G, GRIDMAX, LAMBDA0, LAMBDA0 * r.get("sample", 1.0))
-    f.upload(re, im, 0)
+    if r["kind"] == 1:
+        b.generate(**kw)
+        f.fill_planes(re, im, 0)
+    else:
+        beam = api.load_beam(**kw)
+        b.upload(beam.x, beam.y, beam.px, beam.py, beam.gamma, beam.theta)
+        f.upload(re, im, 0)
     f.init_solver(r["solver"])
     run = api.Run(ctx, b, f, st, LAMBDA0, r.get("sample", 1.0), r["kind"] == 1, False, 1)
     run.advance(len(st))
