@@ -94,7 +94,8 @@ def measured_traffic(cls: str):
     return None
 
 
-def run_gpu(prec: int, beam, steps, ngrid: int, gridmax: float, warmup: int, nsteps: int):
+def run_gpu(prec: int, beam, steps, ngrid: int, gridmax: float, warmup: int, nsteps: int,
+            solver=None):
     from genesis4_cuda_port_amd import api
     from genesis4_cuda_port_amd._abi import KERNEL_CLASSES
     nslice = beam.offsets.size - 1
@@ -102,7 +103,7 @@ def run_gpu(prec: int, beam, steps, ngrid: int, gridmax: float, warmup: int, nst
         b = api.Beam(ctx, prec, beam.offsets, beam.current, GAMMA0)
         b.upload(beam.x, beam.y, beam.px, beam.py, beam.gamma, beam.theta)
         f = api.Field(ctx, prec, nslice, ngrid, gridmax, LAMBDA0, LAMBDA0)
-        f.init_solver(api.FFT)
+        f.init_solver(api.FFT if solver is None else solver)
         nrun = warmup + nsteps
         run = api.Run(ctx, b, f, list(steps)[:nrun], LAMBDA0, 1.0, True, False, 1)
         run.advance(warmup)
@@ -164,6 +165,27 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int):
             "kind": "port",
             "sample": f"FP64 oracle, first {nslice_cpu} of 4096 slices x {npart} particles, "
                       f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
+
+
+def time_dependent_adi(nsteps: int, warmup: int):
+    """Time-dependent SASE in FP64 with the ADI solver: 1024 slices x 8192
+    particles/slice, ngrid 151, gridmax 0.3 mm, flat-top 3 kA, shot noise
+    (seed 1234568), same undulator and lattice as the main workload."""
+    from genesis4_cuda_port_amd import api
+    nslice, npart, ngrid, gm = 1024, 8192, 151, 0.3e-3
+    beam = api.load_beam(nslice=nslice, npart=npart, nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+                         emitn=0.4e-6, beta=15.0, slicelength=LAMBDA0, current=3000.0,
+                         shotnoise=True, seed=1234568)
+    steps = api.lattice_fodo(LAMBDAU, [AW] * 10, 100, 46, 3, 1.5, 1)
+    t, kern, _, _, _ = run_gpu(api.FP64, beam, steps, ngrid, gm, warmup, nsteps, solver=api.ADI)
+    np_ = nslice * npart
+    cells = nslice * ngrid * ngrid
+    return {"workload": f"time-dependent SASE, ADI, FP64, {nslice} slices x {npart} "
+                        f"particles/slice, ngrid {ngrid}",
+            "value": np_ * nsteps / t, "unit": "particle-slice-steps/s",
+            "ms_per_step": t / nsteps * 1e3,
+            "kernels": {k: dict(v, gbs=algorithmic_bytes(k, api.FP64, np_, cells) /
+                                (v["avg_ms"] * 1e-3) / 1e9) for k, v in kern.items()}}
 
 
 def taper_scan(prec: int, nruns: int, seed: int):
@@ -273,6 +295,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adi", action="store_true")
     ap.add_argument("--batch-runs", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1536)
@@ -371,6 +394,9 @@ def main() -> int:
         log("cpu baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
                                            a.cpu_steps)
+    if not a.no_adi:
+        log("time-dependent ADI run (FP64)")
+        out["time_dependent_adi_fp64"] = time_dependent_adi(a.steps, a.warmup)
     if not a.no_batch:
         scan = {"workload": f"taper scan of {a.batch_runs} steady-state runs in one batch "
                             "(uneven grid and particle count, 10 x 100-period modules, ADI)",

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <set>
@@ -175,7 +176,9 @@ struct Deck {
   std::vector<int32_t> offsets;
   std::vector<double> cur;
   double zdone = 0;  // lattice position reached by earlier &track namelists
+  int stepsdone = 0;  // integration steps of earlier &track namelists
   int nrecords = 0;
+  int ndumps = 0;
 };
 
 void do_setup(Deck& d, const Namelist& nl) {
@@ -341,6 +344,47 @@ void create_state(Deck& d, RunState& rs, int device) {
   check(g4h_field_upload(rs.field, re.data(), im.data(), 0));
 }
 
+// State after integration step `gstep` (lattice position z) in
+// out/Dump/<gstep>/: the field in logical slice order (rows: slices,
+// columns: cells ix + iy*ngrid) as field_re.npy / field_im.npy, and the
+// particles (rows: slices, columns: particles of the slice) as
+// beam_<x|y|px|py|gamma|theta>.npy, with z and the kinds in dump.json.
+void write_dump(Deck& d, RunState& rs, const std::string& out, int gstep, double z, bool field,
+                bool beam) {
+  const std::string dir = out + "/Dump/" + std::to_string(gstep);
+  make_dirs(dir);
+  const int S = d.nslice;
+  if (field) {
+    const size_t plane = static_cast<size_t>(d.ngrid) * d.ngrid;
+    std::vector<double> re(plane * S), im(plane * S), lre(plane * S), lim(plane * S);
+    int first = 0;
+    check(g4h_field_download(rs.field, re.data(), im.data(), &first));
+    for (int is = 0; is < S; ++is) {
+      const size_t p = static_cast<size_t>((is + first) % S) * plane;
+      std::copy(re.begin() + p, re.begin() + p + plane, lre.begin() + static_cast<size_t>(is) * plane);
+      std::copy(im.begin() + p, im.begin() + p + plane, lim.begin() + static_cast<size_t>(is) * plane);
+    }
+    write_npy(dir + "/field_re.npy", lre, S, static_cast<int>(plane));
+    write_npy(dir + "/field_im.npy", lim, S, static_cast<int>(plane));
+  }
+  if (beam) {
+    const size_t np = static_cast<size_t>(d.setup.npart) * S;
+    std::vector<double> a[6];
+    for (auto& v : a) v.resize(np);
+    check(g4h_beam_download(rs.beam, a[0].data(), a[1].data(), a[2].data(), a[3].data(),
+                            a[4].data(), a[5].data()));
+    const char* names[6] = {"x", "y", "px", "py", "gamma", "theta"};
+    for (int c = 0; c < 6; ++c)
+      write_npy(dir + "/beam_" + names[c] + ".npy", a[c], S, d.setup.npart);
+  }
+  std::ofstream js(dir + "/dump.json");
+  js << "{\"step\": " << gstep << ", \"z\": " << std::setprecision(17) << z
+     << ", \"field\": " << (field ? "true" : "false") << ", \"beam\": " << (beam ? "true" : "false")
+     << ", \"nslice\": " << S << ", \"ngrid\": " << d.ngrid << ", \"npart\": " << d.setup.npart
+     << "}\n";
+  ++d.ndumps;
+}
+
 void write_records(const Deck& d, const std::string& out, const std::vector<double>& zs,
                    const std::vector<double>& braw, const std::vector<double>& fraw, int nrec,
                    int nharm, const std::vector<g4h_step>& steps) {
@@ -440,9 +484,12 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   const bool periodic = k.b("periodic", false);
   const int nharm = std::max(1, k.i("bunchharm", 1));
   const std::string prec = k.s("precision", d.prec == G4H_FP64 ? "fp64" : "mixed");
-  if (k.i("field_dump_step", 0) != 0 || k.i("beam_dump_step", 0) != 0 ||
-      k.i("sort_step", 0) != 0 || k.b("field_dump_at_undexit", false))
-    throw Unsupported("&track: dumps and sorting are not supported");
+  const int field_dump_step = k.i("field_dump_step", 0);
+  const int beam_dump_step = k.i("beam_dump_step", 0);
+  const bool dump_at_exit = k.b("field_dump_at_undexit", false);
+  if (field_dump_step < 0 || beam_dump_step < 0)
+    throw std::runtime_error("&track dump steps must be >= 0");
+  if (k.i("sort_step", 0) != 0) throw Unsupported("&track: sorting is not supported");
   if (k.b("exclusive_harmonics", false))
     throw Unsupported("&track exclusive_harmonics is not supported");
   k.get("s0");
@@ -476,8 +523,29 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   check(g4h_run_create(rs.ctx, rs.beam, rs.field, steps.data(), static_cast<int>(steps.size()),
                        &cfg, &run));
   std::unique_ptr<g4h_run, void (*)(g4h_run*)> guard(run, g4h_run_destroy);
-  int done = 0;
-  check(g4h_run_advance(run, static_cast<int>(steps.size()), &done));
+  // Dumps of the field and / or beam after a step: lattice markers (bit 1
+  // field, bit 2 beam; Gencore.cpp:176-181 applies a step's marker before
+  // the next step, and the last one after the loop, :276), every
+  // field_dump_step / beam_dump_step integration steps, and the field at
+  // each undulator exit.  The run advances to each such step, then the state
+  // is copied out.  Sorting (marker bit 4) is not built.
+  int done = 0, at = 0;
+  const int ns = static_cast<int>(steps.size());
+  for (int i = 0; i < ns; ++i) {
+    const g4h_step& st = steps[static_cast<size_t>(i)];
+    if (st.marker & 4) throw Unsupported("lattice marker sort = 1: sorting is not supported");
+    const int gstep = d.stepsdone + i + 1;
+    const bool exit_und =
+        st.in_undulator && (i + 1 == ns || !steps[static_cast<size_t>(i) + 1].in_undulator);
+    const bool fd = (st.marker & 1) || (field_dump_step > 0 && gstep % field_dump_step == 0) ||
+                    (dump_at_exit && exit_und);
+    const bool bd = (st.marker & 2) || (beam_dump_step > 0 && gstep % beam_dump_step == 0);
+    if (!fd && !bd) continue;
+    check(g4h_run_advance(run, i + 1 - at, &done));
+    at = i + 1;
+    write_dump(d, rs, out, gstep, st.z, fd, bd);
+  }
+  check(g4h_run_advance(run, ns - at, &done));
   int nrec = 0;
   check(g4h_run_records(run, &nrec));
   const int S = d.nslice;
@@ -489,6 +557,7 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
     if (st.outstep) zs.push_back(st.z);
   zs.resize(static_cast<size_t>(nrec));
   d.zdone = steps.back().z;
+  d.stepsdone += ns;
   const std::string dir = out + (d.nrecords > 0 ? "/track" + std::to_string(d.nrecords) : "");
   write_records(d, dir, zs, braw, fraw, nrec, nharm, steps);
   d.nrecords += nrec;

@@ -77,7 +77,7 @@ typedef struct g4h_step {
   double reserved;   /* keeps the struct at 22 doubles; must be 0          */
   int32_t helical;   /* 1 = helical undulator, 0 = planar                  */
   int32_t in_undulator; /* 1 if the field is driven by the beam this step   */
-  int32_t marker;    /* marker bits (1 dump beam, 2 dump field, 4 sort)    */
+  int32_t marker;    /* marker bits (1 dump field, 2 dump beam, 4 sort, 8 stop) */
   int32_t outstep;   /* 1 if diagnostics are recorded after this step       */
 } g4h_step;
 

@@ -94,3 +94,32 @@ def test_g4run_cli_writes_the_same_records(tmp_path):
         a = np.load(tmp_path / "cli" / f"{name}.npy")
         b = np.load(tmp_path / "lib" / f"{name}.npy")
         assert relerr(a, b) < 1e-12, name
+
+
+def test_deck_dumps_match_records(tmp_path):
+    """&track field_dump_step / beam_dump_step write the state after those
+    integration steps; a field dump gives the slice power of the record of
+    the same step, a beam dump the slice energy and particle count."""
+    print("case deck dumps", flush=True)
+    deck = _deck(tmp_path, "field_dump_step=10\nbeam_dump_step=15")
+    out = tmp_path / "out"
+    nrec = api.deck_run(deck, out)
+    dumps = sorted(int(p.name) for p in (out / "Dump").iterdir())
+    assert dumps == [k for k in range(1, nrec) if k % 10 == 0 or k % 15 == 0]
+    gp = np.load(out / "Field" / "power.npy")
+    ge = np.load(out / "Beam" / "energy.npy")
+    dgrid = 2 * 120e-6 / 126
+    ks = 2 * np.pi / LAMBDA0
+    for k in dumps:
+        meta = json.loads((out / "Dump" / str(k) / "dump.json").read_text())
+        assert meta["step"] == k and meta["field"] == (k % 10 == 0)
+        if meta["field"]:
+            re = np.load(out / "Dump" / str(k) / "field_re.npy")
+            im = np.load(out / "Dump" / str(k) / "field_im.npy")
+            assert re.shape == (NSLICE, 127 * 127)
+            p = np.sum(re * re + im * im, axis=1) * (dgrid * 510999.06 / ks) ** 2 / 376.73
+            assert relerr(p, gp[k]) < 1e-10
+        if meta["beam"]:
+            g = np.load(out / "Dump" / str(k) / "beam_gamma.npy")
+            assert g.shape == (NSLICE, 4800)
+            assert relerr(g.mean(axis=1), ge[k]) < 1e-12
