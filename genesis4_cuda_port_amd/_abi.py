@@ -168,6 +168,8 @@ PROTOTYPES = {
     "g4h_run_create": (C.c_int, [_vp, _vp, _vp, C.POINTER(Step), C.c_int,
                                  C.POINTER(RunConfig), C.POINTER(_vp)]),
     "g4h_run_destroy": (None, [_vp]),
+    "g4h_run_capture": (C.c_int, [_vp]),
+    "g4h_run_replay": (C.c_int, [_vp]),
     "g4h_run_advance": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "g4h_run_records": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "g4h_run_download_diag": (C.c_int, [_vp, _dp, _dp]),

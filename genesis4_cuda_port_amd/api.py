@@ -282,6 +282,14 @@ class Run:
         check(lib().g4h_run_advance(self.h, nsteps, C.byref(done)))
         return int(done.value)
 
+    def capture(self):
+        """Capture the whole run into a hipGraph (after one full advance)."""
+        check(lib().g4h_run_capture(self.h))
+
+    def replay(self):
+        """Run again from re-initialised beam and field, as one graph launch."""
+        check(lib().g4h_run_replay(self.h))
+
     def records(self) -> int:
         n = C.c_int()
         check(lib().g4h_run_records(self.h, C.byref(n)))

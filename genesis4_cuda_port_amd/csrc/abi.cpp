@@ -79,6 +79,15 @@ void device_free(Context* ctx, void* p) {
 
 const void* Context::param_push(const void* data, size_t bytes) {
   require(bytes <= kParamBytes, "parameter block too large");
+  if (capturing) {
+    require(arena && arena->used < arena->slots, "parameter arena of the capture is full");
+    char* h = arena->host + static_cast<size_t>(arena->used) * kParamBytes;
+    char* d = arena->dev + static_cast<size_t>(arena->used) * kParamBytes;
+    ++arena->used;
+    std::memcpy(h, data, bytes);
+    G4_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+    return d;
+  }
   if (!param_host) {
     void* h = nullptr;
     G4_HIP(hipHostMalloc(&h, kParamSlots * kParamBytes, hipHostMallocDefault));
@@ -104,7 +113,7 @@ const void* Context::param_push(const void* data, size_t bytes) {
 }
 
 void Context::param_commit() {
-  if (param_last < 0) return;
+  if (capturing || param_last < 0) return;
   G4_HIP(hipEventRecord(param_ev[param_last], stream));
   param_pending[param_last] = true;
   param_last = -1;
@@ -697,6 +706,7 @@ g4h_status g4h_field_init_solver(g4h_field* f, g4h_solver_kind kind, int filter,
   f->fft_tw.release();  // tables are rebuilt on the next init
   f->fft_sigmoid.release();
   f->delz_tables.clear();
+  ++f->generation;  // device tables of the solver are gone
   f->adi_c = f->adi_cbet = f->adi_cwet = nullptr;
   f->fft_prop = nullptr;
   f->adi_delz = 0.0;

@@ -140,6 +140,10 @@ struct Context {
   // context while some are alive defers the release until the last one goes.
   int children = 0;
   bool destroy_pending = false;
+  // stream capture into a hipGraph in progress on this context's stream:
+  // no profiling events, no tuner trials, no allocations, parameter blocks
+  // from the capture arena
+  bool capturing = false;
   // profiling
   bool profile = false;
   struct Prof {
@@ -158,7 +162,7 @@ struct Context {
   // Bracket a launch with events when profiling is on.
   template <class F>
   void launch(int cls, F&& f) {
-    if (!profile) {
+    if (!profile || capturing) {
       f();
       G4_HIP(hipGetLastError());
       return;
@@ -189,6 +193,15 @@ struct Context {
   const void* param_push(const void* data, size_t bytes);
   void param_commit();
   void param_release();
+  // While capturing, blocks go to an arena that lives as long as the graph
+  // (a replay reads them again): pinned host and device slots, filled in
+  // order.
+  struct ParamArena {
+    char* host = nullptr;
+    char* dev = nullptr;
+    int slots = 0, used = 0;
+  };
+  ParamArena* arena = nullptr;
 
   Tuner tuner;
   // Variant of `family` for `shape` to launch now; while measuring, *a and *b
@@ -196,6 +209,7 @@ struct Context {
   int tune_pick(int family, const std::vector<int>& shape, int nvar, hipEvent_t* a,
                 hipEvent_t* b);
   void tune_release();  // destroys the events of undecided entries
+  void tune_settle();   // decides every entry whose trials have completed
 };
 
 // Records the tuner's events around the launches of one family call.
@@ -300,6 +314,7 @@ struct Field {
     DevBuf<double2> dev;
   };
   std::map<double, DelzTable> delz_tables;
+  int generation = 0;  // bumped when device tables a captured run may use are released
   // ADI
   double adi_delz = 0.0;
   const double2* adi_c = nullptr;  // current tables [ngrid] (views into delz_tables)

@@ -13,6 +13,12 @@
 //      incoming plane);
 //   4. on output steps, the map of per-plane field sums to logical slices.
 // Diagnostics stay in device history buffers until g4h_run_download_diag.
+//
+// Ensembles (the same lattice and shapes run again with a new beam or seed:
+// shot-noise statistics, scans) replay the run from a hipGraph: after one
+// eager pass, g4h_run_capture records the initial record and every step of
+// the run once; g4h_run_replay then enqueues the whole run as one graph
+// launch on state the caller has re-initialised.
 #include <cmath>
 #include <vector>
 
@@ -36,6 +42,22 @@ struct g4h_run {
   int nrec = 0;
   int cap = 0;
   DevBuf<double> beam_hist, field_hist;
+  // field ring state when the run starts, and the captured run
+  int first0 = 0;
+  double acc0 = 0.0;
+  int end_first = 0, end_nrec = 0;
+  double end_acc = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int field_generation = 0;  // of the field when captured
+  Context::ParamArena arena;
+
+  ~g4h_run() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (arena.host) (void)hipHostFree(arena.host);
+    if (arena.dev) (void)hipFree(arena.dev);
+  }
 
   size_t beam_rec() const { return static_cast<size_t>(b->nslice) * G4H_BEAM_RAW(cfg.nharm); }
   size_t field_rec() const { return static_cast<size_t>(f->nslice) * G4H_FIELD_RAW; }
@@ -108,6 +130,8 @@ g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,
     r->f = field;
     r->steps.assign(steps, steps + nsteps);
     r->cfg = *cfg;
+    r->first0 = field->first;
+    r->acc0 = field->accuslip;
     int nout = 0;
     for (int i = 0; i < nsteps; ++i) nout += steps[i].outstep ? 1 : 0;
     r->cap = cfg->record_diag ? 1 + nout : 0;
@@ -142,6 +166,81 @@ g4h_status g4h_run_advance(g4h_run* r, int nsteps, int* done) {
   const int end = std::min<int>(static_cast<int>(r->steps.size()), r->done + nsteps);
   for (; r->done < end; ++r->done) r->step(r->steps[r->done]);
   if (done) *done = r->done;
+  G4_API_END
+}
+
+g4h_status g4h_run_capture(g4h_run* r) {
+  G4_API_BEGIN
+  require(r, "run is null");
+  require(!r->exec, "the run is already captured");
+  require(r->done == static_cast<int>(r->steps.size()),
+          "advance the run to its end once before capturing it");
+  Context* ctx = r->ctx;
+  ctx->activate();
+  G4_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->tune_settle();
+  if (r->b->prec == G4H_FP64) {  // one parameter block per particle launch
+    const int slots = static_cast<int>(r->steps.size()) + 1;
+    void* h = nullptr;
+    void* d = nullptr;
+    G4_HIP(hipHostMalloc(&h, static_cast<size_t>(slots) * Context::kParamBytes, hipHostMallocDefault));
+    r->arena.host = static_cast<char*>(h);
+    G4_HIP(hipMalloc(&d, static_cast<size_t>(slots) * Context::kParamBytes));
+    r->arena.dev = static_cast<char*>(d);
+    r->arena.slots = slots;
+  }
+  r->end_first = r->f->first;
+  r->end_acc = r->f->accuslip;
+  r->end_nrec = r->nrec;
+  r->done = 0;
+  r->nrec = 0;
+  r->f->first = r->first0;
+  r->f->accuslip = r->acc0;
+  ctx->capturing = true;
+  ctx->arena = &r->arena;
+  G4_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    r->record_initial();
+    for (const g4h_step& st : r->steps) r->step(st);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(ctx->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    ctx->capturing = false;
+    ctx->arena = nullptr;
+    r->done = static_cast<int>(r->steps.size());
+    r->nrec = r->end_nrec;
+    r->f->first = r->end_first;
+    r->f->accuslip = r->end_acc;
+    throw;
+  }
+  ctx->capturing = false;
+  ctx->arena = nullptr;
+  G4_HIP(hipStreamEndCapture(ctx->stream, &r->graph));
+  G4_HIP(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+  r->field_generation = r->f->generation;
+  r->done = static_cast<int>(r->steps.size());
+  require(r->nrec == r->end_nrec && r->f->first == r->end_first,
+          "captured run differs from the eager run");
+  G4_API_END
+}
+
+g4h_status g4h_run_replay(g4h_run* r) {
+  G4_API_BEGIN
+  require(r, "run is null");
+  require(r->exec, "capture the run first (g4h_run_capture)");
+  require(r->f->generation == r->field_generation,
+          "the field's solver was re-initialised after the capture");
+  require(r->f->first == r->first0,
+          "re-initialise the field (upload or fill with the run's initial ring offset) before "
+          "replaying");
+  r->ctx->activate();
+  r->f->accuslip = r->acc0;
+  G4_HIP(hipGraphLaunch(r->exec, r->ctx->stream));
+  r->done = static_cast<int>(r->steps.size());
+  r->nrec = r->end_nrec;
+  r->f->first = r->end_first;
+  r->f->accuslip = r->end_acc;
   G4_API_END
 }
 

@@ -50,6 +50,7 @@ int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipE
       return e.choice;
     }
   }
+  if (capturing) return 0;  // no trials inside a graph: variant 0 until decided
   const int total = Tuner::kReps * e.nvar;
   if (e.issued < total) {
     // next trial: variants in turn, so every variant sees warm and cold caches
@@ -83,6 +84,16 @@ int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipE
     g_decided[gkey] = e.choice;
   }
   return e.choice;
+}
+
+void Context::tune_settle() {
+  for (auto& kv : tuner.table) {
+    Tuner::Entry& e = kv.second;
+    if (e.choice >= 0 || e.issued < Tuner::kReps * e.nvar) continue;
+    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<int> shape(kv.first.begin() + 1, kv.first.end());
+    (void)tune_pick(kv.first[0], shape, e.nvar, &a, &b);  // decides if complete
+  }
 }
 
 void Context::tune_release() {

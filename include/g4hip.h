@@ -222,6 +222,15 @@ void g4h_run_destroy(g4h_run* run);
 /* Execute up to `nsteps` further lattice steps (asynchronous). *done
  * (may be NULL) receives the number of steps executed so far. */
 g4h_status g4h_run_advance(g4h_run* run, int nsteps, int* done);
+/* Ensembles: after the run has been advanced to its end once, capture its
+ * whole sequence (initial record and every step) into a hipGraph; then, after
+ * re-initialising the beam (g4h_beam_upload / g4h_beam_generate) and the
+ * field (g4h_field_upload / g4h_field_fill_planes with the run's initial ring
+ * offset), g4h_run_replay enqueues the whole run again as one graph launch
+ * and the records are read with g4h_run_download_diag as before.  No
+ * counterpart in the reference (one run per process). */
+g4h_status g4h_run_capture(g4h_run* run);
+g4h_status g4h_run_replay(g4h_run* run);
 /* Number of diagnostic records written so far. */
 g4h_status g4h_run_records(const g4h_run* run, int* nrecords);
 /* Blocking copy of records [0, nrecords) (layouts as g4h_diag_*_raw). */

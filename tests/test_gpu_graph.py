@@ -1,0 +1,74 @@
+"""GPU: ensembles replayed from a captured hipGraph (g4h_run_capture /
+g4h_run_replay).
+
+One run is advanced eagerly and captured; it is then replayed several
+times, alternating between two ensemble members (beam and seed field of
+different seeds), with eager work and synchronisations in between.  Every
+replay must give the records of the same member run eagerly in a fresh
+context: FP64 within 1e-12 relative (the deposition sums may add in another
+order), MIXED within 1e-5.  Replaying without re-initialising the field is
+refused."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+from cases import LAMBDA0
+from genesis4_cuda_port_amd import api
+from genesis4_cuda_port_amd._abi import G4Error
+
+pytestmark = pytest.mark.gpu
+
+from test_gpu_parity import gpu_beam, gpu_field, relerr  # noqa: E402
+
+
+def _eager(prec, kind, c):
+    with api.Context(0) as ctx:
+        b = gpu_beam(ctx, c, prec)
+        f = gpu_field(ctx, c, prec, kind=kind)
+        run = api.Run(ctx, b, f, c.steps, LAMBDA0, c.sample, c.timerun, False, 2)
+        run.advance(len(c.steps))
+        out = run.download_diag()
+        run.close()
+        f.close()
+        b.close()
+    return out
+
+
+@pytest.mark.parametrize("prec,kind", [(api.FP64, api.ADI), (api.FP64, api.FFT),
+                                       (api.MIXED, api.FFT), (api.MIXED, api.ADI)])
+def test_graph_replay_matches_eager(prec, kind):
+    print(f"case graph_replay prec={prec} kind={kind}", flush=True)
+    g = 32 if kind == api.FFT else 31
+    members = [cases.make_case(nslice=8, npart=1024, ngrid=g, nsteps=20, seed_power=1e4,
+                               seed=seed) for seed in (1234568, 4321)]
+    members[1].seed_re *= 0.5
+    ref = [_eager(prec, kind, m) for m in members]
+    tol = 1e-12 if prec == api.FP64 else 1e-5
+    with api.Context(0) as ctx:
+        c = members[0]
+        b = gpu_beam(ctx, c, prec)
+        f = gpu_field(ctx, c, prec, kind=kind)
+        run = api.Run(ctx, b, f, c.steps, LAMBDA0, c.sample, c.timerun, False, 2)
+        run.advance(len(c.steps))
+        first = run.download_diag()
+        assert relerr(first[0], ref[0][0]) < tol and relerr(first[1], ref[0][1]) < tol
+        run.capture()
+        with pytest.raises(G4Error):  # the field still holds the end of the run
+            run.replay()
+        for rep, k in enumerate((1, 0, 1)):
+            m = members[k]
+            b.upload(m.beam.x, m.beam.y, m.beam.px, m.beam.py, m.beam.gamma, m.beam.theta)
+            f.upload(m.seed_re, m.seed_im, 0)
+            run.replay()
+            braw, fraw = run.download_diag()
+            eb, ef = relerr(braw, ref[k][0]), relerr(fraw, ref[k][1])
+            print(f"  replay {rep} (member {k}): beam {eb:.3e} field {ef:.3e}", flush=True)
+            assert eb < tol and ef < tol
+            # eager work on the same stream between replays
+            b.diag_raw(2)
+            ctx.synchronize()
+        run.close()
+        f.close()
+        b.close()
