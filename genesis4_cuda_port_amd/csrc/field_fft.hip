@@ -350,7 +350,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
       constexpr int r = decltype(ic)::value;
       d[r].x = 0;
       d[r].y = 0;
-      if (ok) d[r] = row[t + L * r];
+      if (ok) d[r] = ld_stream(&row[t + L * r]);
     });
   };
   int64_t grp = blockIdx.x * static_cast<int64_t>(C::kWaves) + wave;
@@ -380,17 +380,17 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
         T2 o = v[r];
         o.x *= sc;
         o.y *= sc;
-        row[t + L * r] = o;
+        st_stream(&row[t + L * r], o);
       });
     }
   }
 }
 
 // Column pass: a workgroup owns kTile adjacent columns of one ring plane;
-// each line slot of its waves loads one column straight from HBM (the
-// workgroup's loads of one row form one contiguous segment), transforms it
-// in registers, and stores it back with the source added and the field sums
-// of the tile reduced.
+// the tile is staged through LDS with whole row segments per instruction
+// (or, for tiles narrower than 64 bytes, each line slot loads its column
+// directly), each wave transforms its columns in registers, and the tile is
+// stored back with the source added and the field sums of the tile reduced.
 template <class P, int G>
 __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     typename C2<P>::T* u, int nslice, const double2* tw_d, const double2* mult_d,
@@ -430,7 +430,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     fftr::static_for<0, G / RPI>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const int row = i * RPI + sc_row;
-      buf[row * TP + sc_col] = plane[static_cast<int64_t>(row) * G + c0 + sc_col];
+      buf[row * TP + sc_col] = ld_stream(&plane[static_cast<int64_t>(row) * G + c0 + sc_col]);
     });
     __syncthreads();
     fftr::static_for<0, R>([&](auto ic) {
@@ -495,15 +495,15 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       o.x *= sc;
       o.y *= sc;
       if (scol && row >= rc.y && row < rc.w) {
-        const T2 sv = sp[gi];
+        const T2 sv = ld_stream(&sp[gi]);
         o.x += P(2) * sv.x;
         o.y += P(2) * sv.y;
       } else if (wp) {
-        const double2 wv = wp[gi];
+        const double2 wv = ld_stream(&wp[gi]);
         o.x += static_cast<P>(wv.x);
         o.y += static_cast<P>(wv.y);
       }
-      plane[gi] = o;
+      st_stream(&plane[gi], o);
       if (partials) {
         const double re = o.x, im = o.y;
         const double wei = re * re + im * im;

@@ -5,6 +5,39 @@
 
 namespace g4 {
 
+// ---- streaming access ----------------------------------------------------
+// Data that a pass reads or writes exactly once (particle state, field
+// planes in the FFT passes, source planes) goes through nontemporal loads
+// and stores, which keep it from displacing the reused data in the caches:
+// a plain 2 GiB device copy reaches ~5.0 TB/s with ordinary and ~5.8 TB/s
+// with nontemporal accesses on MI355X.
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(float2* p, float2 v) {
+  nt_f2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<nt_f2*>(p));
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+  nt_d2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<nt_d2*>(p));
+}
+
 // ---- particle pass -------------------------------------------------------
 enum Stage : int {
   ST_T1 = 1,     // first transverse half step (with chicane)

@@ -688,13 +688,13 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   if (st & ST_R56) th += prm.r56 * gshift(g, g0);
   if (st & ST_T2) transverse<P>(prm.t2, g0, g, x, y, px, py, false);
   if (st & (ST_T1 | ST_T2)) {
-    b.x[i] = x;
-    b.y[i] = y;
-    b.px[i] = px;
-    b.py[i] = py;
+    st_stream(&b.x[i], x);
+    st_stream(&b.y[i], y);
+    st_stream(&b.px[i], px);
+    st_stream(&b.py[i], py);
   }
-  if (st & ST_PUSH) b.g[i] = g;
-  if (st & (ST_PUSH | ST_R56)) b.th[i] = th;
+  if (st & ST_PUSH) st_stream(&b.g[i], g);
+  if (st & (ST_PUSH | ST_R56)) st_stream(&b.th[i], th);
 
   // sine and cosine of the final phase, shared by the deposition (harm 1)
   // and the h = 1 bunching sum
@@ -1000,12 +1000,12 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   P nx = P(0), ny = P(0), npx = P(0), npy = P(0), ngm = P(0);
   double nth = 0.0;
   if (have) {
-    nx = b.x[i];
-    ny = b.y[i];
-    npx = b.px[i];
-    npy = b.py[i];
-    ngm = b.g[i];
-    nth = b.th[i];
+    nx = ld_stream(&b.x[i]);
+    ny = ld_stream(&b.y[i]);
+    npx = ld_stream(&b.px[i]);
+    npy = ld_stream(&b.py[i]);
+    ngm = ld_stream(&b.g[i]);
+    nth = ld_stream(&b.th[i]);
   }
   while (have) {
     const P x = nx, y = ny, px = npx, py = npy, g = ngm;
@@ -1013,12 +1013,12 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     const int64_t inext = i + kThreads;
     const bool hnext = inext < i1;
     if (hnext) {
-      nx = b.x[inext];
-      ny = b.y[inext];
-      npx = b.px[inext];
-      npy = b.py[inext];
-      ngm = b.g[inext];
-      nth = b.th[inext];
+      nx = ld_stream(&b.x[inext]);
+      ny = ld_stream(&b.y[inext]);
+      npx = ld_stream(&b.px[inext]);
+      npy = ld_stream(&b.py[inext]);
+      ngm = ld_stream(&b.g[inext]);
+      nth = ld_stream(&b.th[inext]);
     }
     process<P, NF, DEP, NH>(prm, mk, s, i, b, x, y, px, py, g, th, scl, src, win, wx0, wy0, ww,
                             wh, s_fb, ps);
