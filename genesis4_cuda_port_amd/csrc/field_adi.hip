@@ -214,8 +214,8 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
       v[r] = make_double2(0.0, 0.0);
       sv[r] = make_double2(0.0, 0.0);
       if (col && k < G) {
-        v[r] = to_d(up[static_cast<int64_t>(k) * G + t]);
-        if (src_col && k >= rc.y && k < rc.w) sv[r] = to_d(sp[static_cast<int64_t>(k) * G + t]);
+        v[r] = to_d(ld_stream(&up[static_cast<int64_t>(k) * G + t]));
+        if (src_col && k >= rc.y && k < rc.w) sv[r] = to_d(ld_stream(&sp[static_cast<int64_t>(k) * G + t]));
       }
     }
 #pragma unroll
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
           double2 rr = rhs_of(v[r], buf[t + 2], buf[t], cs);
           if (src_col && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
           yprev = zmul(zsub(rr, zmul(cl[k], yprev)), cb[k]);
-          yp[static_cast<int64_t>(k) * G + t] = yprev;
+          st_stream(&yp[static_cast<int64_t>(k) * G + t], yprev);
         }
       }
     }
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       const int k = k0 + r;
-      y[r] = (col && k < G) ? yp[static_cast<int64_t>(k) * G + t] : make_double2(0.0, 0.0);
+      y[r] = (col && k < G) ? ld_stream(&yp[static_cast<int64_t>(k) * G + t]) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int r = kR - 1; r >= 0; --r) {
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
         o.x = static_cast<P>(uu.x);
         o.y = static_cast<P>(uu.y);
         const int64_t gi = static_cast<int64_t>(k) * G + t;
-        up[gi] = o;
+        st_stream(&up[gi], o);
         if (stats) {
           const double re = o.x, im = o.y;
           const double wei = re * re + im * im;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
       ns[r] = tz;
       if (k < G) {
         const int64_t rowoff = static_cast<int64_t>(k) * G;
-        if (act) nv[r] = up[rowoff + cx];
+        if (act) nv[r] = ld_stream(&up[rowoff + cx]);
         if (eact) ne[r] = up[rowoff + cex];
         if (scol && k >= rc.y && k < rc.w) ns[r] = sp[rowoff + cx];
       }
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
         double2 rr = rhs_of(v[r], right, left, cs);
         if (scol && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
         yprev = zmul(zsub(rr, zmul(c_d[k], yprev)), cbet_d[k]);
-        if (act) yp[static_cast<int64_t>(k) * G + cx] = yprev;
+        if (act) st_stream(&yp[static_cast<int64_t>(k) * G + cx], yprev);
       }
     }
   }
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
     for (int r = 0; r < kCh; ++r) {
       const int k = k0 + r;
       ny[r] = zero;
-      if (act && k < G) ny[r] = yp[static_cast<int64_t>(k) * G + cx];
+      if (act && k < G) ny[r] = ld_stream(&yp[static_cast<int64_t>(k) * G + cx]);
     }
   };
   load_y(((G - 1) / kCh) * kCh);
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
         o.x = static_cast<P>(uu.x);
         o.y = static_cast<P>(uu.y);
         const int64_t gi = static_cast<int64_t>(k) * G + cx;
-        up[gi] = o;
+        st_stream(&up[gi], o);
         if (partials) {
           const double re = o.x, im = o.y;
           const double wei = re * re + im * im;
