@@ -215,6 +215,52 @@ __device__ __forceinline__ void plane_step(int mode, double str, double off, dou
 __device__ __forceinline__ double psqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float psqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 
+// ---- FP64 reciprocals and square roots ------------------------------------
+// 1/v and 1/sqrt(v) from the hardware estimates and two Newton steps, to
+// within an ulp: for quantities that no cancellation amplifies (transverse
+// maps, deposition weights, dgamma/dz), and as the approximate reciprocals
+// of the correctly rounded forms below.
+__device__ __forceinline__ double rcp_nr(double v) {
+  double y = __builtin_amdgcn_rcp(v);
+  double e = fma(-v, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-v, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_nr(double v) {
+  double y = __builtin_amdgcn_rsq(v);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(0.5 * y, fma(-v * y, y, 1.0), y);
+  return y;
+}
+
+// The reference's q = 1 / fl(sqrt(fl(1 - fl(a / b)))) with every operation
+// correctly rounded (BeamSolverCUDA.cpp:145-223): theta' = xks (1 - q) + xku
+// cancels ~8 digits, so q must round exactly as there.  For b = gamma^2 the
+// square root is of u = 1 - d with d ~ aw^2/gamma^2 tiny, and both it and
+// the reciprocal lie next to one.  Each result is an estimate within a few
+// ulp (a * rb with rb ~ 1/b; the series 1 - d/2 - d^2/8 and 1 + e + e^2)
+// followed by one Newton correction whose residual is exact in an FMA
+// (a - q b, u - r^2, 1 - r q) and whose one rounding is then the correctly
+// rounded result: the error left before it is ~(estimate error)^2, far
+// below the distance to a rounding boundary except for a fraction ~2^-50
+// of inputs.  Outside 0 <= d <= 2^-16 the division, sqrt and reciprocal
+// are taken as such.
+__device__ __forceinline__ double inv_btpar(double a, double b, double rb) {
+  const double q1 = a * rb;
+  const double x = fma(fma(-q1, b, a), rb, q1);  // fl(a / b)
+  const double u = 1.0 - x;
+  const double d = 1.0 - u;  // exact for u in [1/2, 2]
+  if (d >= 0.0 && d <= 0x1p-16) {
+    const double r0 = 1.0 - d * fma(d, 0.125, 0.5);
+    const double r = fma(fma(-r0, r0, u), fma(0.25, d, 0.5), r0);  // fl(sqrt(u))
+    const double e = 1.0 - r;                                      // exact
+    const double q0 = 1.0 + fma(e, e, e);
+    return fma(fma(-r, q0, 1.0), q0, q0);  // fl(1 / r)
+  }
+  return 1.0 / sqrt(1.0 - a / b);
+}
+
 // gamma^2 - 1 - extra (extra = aw^2 + px^2 + py^2 or px^2 + py^2)
 __device__ __forceinline__ double gz2(double g, double g0, double g2c, double aw2, bool with_aw) {
   (void)g0;
@@ -244,8 +290,9 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
     py += P(t.cy);
   }
   if constexpr (sizeof(P) == 8) {
-    const double gz = psqrt(gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py);
-    const double igz = 1.0 / gz;
+    const double gzz = gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py;
+    const double igz = rsq_nr(gzz);  // 1/gammaz
+    const double gz = gzz * igz;
     plane_step(t.xmode, t.xstr, t.xoff, t.delz, x, px, gz, igz, t.xpoly != 0);
     plane_step(t.ymode, t.ystr, t.yoff, t.delz, y, py, gz, igz, t.ypoly != 0);
   } else {
@@ -262,22 +309,30 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
 // ---- particle -> cell (BeamSolverCUDA.cpp:99-116, FieldSolverADICUDA.cpp:
 // 126-143): lower-left cell (ix, iy) and the weight wx, wy of that cell's
 // column/row.  The gather, the deposition and g4h_beam_cell_index share it.
-// FP64 evaluates the reference's (x + gm)/dg.  FP32 uses gm/dg = (ng-1)/2
+// FP64 with EXACT (the cell index) evaluates the reference's (x + gm)/dg;
+// the gather and the deposition take (x + gm) * (1/dg), within an ulp of it
+// (a particle that this moves across a cell edge has weight 0 or 1 on the
+// edge, which both cells agree on).  FP32 uses gm/dg = (ng-1)/2
 // exactly and the correctly rounded 1/dg: the cell coordinate is one FMA and
 // the in-cell fraction a second FMA against the exact half-integer
 // (ng-1)/2 - ix, so the weights keep FP32 precision even where the
 // coordinate itself is ~ng/2 (a direct FP32 (x+gm)/dg would leave only
 // ~2^-24 * ng/2 of a cell, 3e-5 at ng = 1024).
-template <class P>
+template <class P, bool EXACT = false>
 __device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, P idg, int ng, int& ix, int& iy,
                                         P& wx, P& wy) {
   if ((x <= -gm) || (x >= gm) || (y <= -gm) || (y >= gm)) return false;
   if constexpr (sizeof(P) == 8) {
-    (void)idg;
-    const P wxr = (x + gm) / dg;
-    const P wyr = (y + gm) / dg;
+    const P wxr = EXACT ? (x + gm) / dg : (x + gm) * idg;
+    const P wyr = EXACT ? (y + gm) / dg : (y + gm) * idg;
     ix = static_cast<int>(floor(wxr));
     iy = static_cast<int>(floor(wyr));
+    if (!EXACT) {
+      // |x| < gm puts the exact coordinate below ng - 1; the product may
+      // round up onto it
+      ix = min(ix, ng - 2);
+      iy = min(iy, ng - 2);
+    }
     if (ix < 0 || ix >= ng - 1 || iy < 0 || iy >= ng - 1) return false;
     wx = P(1) + static_cast<P>(ix) - wxr;
     wy = P(1) + static_cast<P>(iy) - wyr;
@@ -361,7 +416,8 @@ __device__ __forceinline__ void sincos_near(double d, double a, double s0, doubl
 // ---- ODE ----------------------------------------------------------------
 // FP64: evaluate_longitudinal_ode_* (BeamSolverCUDA.cpp:145-223); the phase
 // factors exp(-i h theta) come from the step's initial ones (bs, bc at
-// theta0 = tthet - dth) through sincos_near.
+// theta0 = tthet - dth) through sincos_near, and 1/beta_par through
+// inv_btpar, rounded as the reference rounds it.
 template <int NF>
 __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, double tthet,
                                       double dth, const double* bs, const double* bc, double btpar,
@@ -378,9 +434,10 @@ __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, 
     cim += ri[i] * c - rr[i] * s;
   }
   const double btper0 = btpar + ztemp1 * cre;
-  const double btpar0 = sqrt(1.0 - btper0 / (tgam * tgam));
-  k2pp += p.xks * (1.0 - 1.0 / btpar0) + p.xku;
-  k2gg += cim / btpar0 / tgam - ez;
+  const double itg = rcp_nr(tgam);
+  const double ib = inv_btpar(btper0, tgam * tgam, itg * itg);  // 1/btpar0
+  k2pp += p.xks * (1.0 - ib) + p.xku;
+  k2gg += cim * ib * itg - ez;
 }
 
 // MIXED: dgamma (FP32) and theta (FP64) with the expansion of the header.
@@ -750,7 +807,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
       P part = P(0);
       if (in) {
         if constexpr (sizeof(P) == 8)
-          part = sqrt(faw2) * scl / g;
+          part = faw2 * rsq_nr(faw2) * scl * rcp_nr(g);
         else
           part = psqrt(faw2) * scl * __builtin_amdgcn_rcpf(float(g0) + g);
       }
@@ -1193,7 +1250,7 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
   if (i >= n) return;
   int ix, iy;
   P wx, wy;
-  cell[i] = cell_of<P>(b.x[i], b.y[i], P(gm), P(dg), P(1.0 / dg), ng, ix, iy, wx, wy)
+  cell[i] = cell_of<P, true>(b.x[i], b.y[i], P(gm), P(dg), P(1.0 / dg), ng, ix, iy, wx, wy)
                 ? ix + iy * ng
                 : -1;
 }
