@@ -161,6 +161,12 @@ class Beam:
         el = None if slice_eloss is None else _dp(_f64(slice_eloss, self.nslice))
         check(lib().g4h_beam_advance(self.ctx.h, self.h, arr, len(fields), C.byref(step), delz, el))
 
+    def set_particle_ez(self, ez=None):
+        """Per-particle longitudinal field of later pushes (beam order, np
+        values, units of slice_eloss); None clears it."""
+        el = None if ez is None else _dp(_f64(ez, self.np))
+        check(lib().g4h_beam_set_particle_ez(self.h, el))
+
     def cell_index(self, field: "Field") -> np.ndarray:
         out = np.empty(self.np, dtype=np.int32)
         check(lib().g4h_beam_cell_index(self.ctx.h, self.h, field.h, _ip(out)))

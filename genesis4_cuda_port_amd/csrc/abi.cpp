@@ -767,6 +767,8 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* b, g4h_field* const* fie
     require(fields[i]->prec == b->prec, "beam and field precision differ");
     require(fields[i]->nslice == b->nslice, "field and beam slice counts differ");
   }
+  require(!(slice_eloss && b->pez_on),
+          "the beam holds a per-particle longitudinal field: fold the slice loss into it");
   ctx->activate();
   const double* d_eloss = nullptr;
   DevBuf<double>& eb = b->eloss;
@@ -781,8 +783,28 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* b, g4h_field* const* fie
   p.stages = ST_PUSH;
   std::vector<Field*> fl(fields, fields + nfield);
   make_push(p.push, *und, b->gamma_ref, delz, fl.data(), nfield, d_eloss);
+  p.push.particle_ez = b->pez_on ? b->pez.get() : nullptr;
   launch_particle(ctx, b, p);
   if (slice_eloss) G4_HIP(hipStreamSynchronize(ctx->stream));
+  G4_API_END
+}
+
+g4h_status g4h_beam_set_particle_ez(g4h_beam* b, const double* ez) {
+  G4_API_BEGIN
+  check_beam(b);
+  Context* ctx = b->ctx;
+  ctx->activate();
+  if (!ez) {
+    b->pez_on = false;
+    return G4H_OK;
+  }
+  // the previous values may still be read by queued passes
+  G4_HIP(hipStreamSynchronize(ctx->stream));
+  if (b->pez.size() != static_cast<size_t>(b->np)) b->pez.alloc(ctx, static_cast<size_t>(b->np));
+  if (b->np > 0)
+    G4_HIP(hipMemcpy(b->pez.get(), ez, sizeof(double) * static_cast<size_t>(b->np),
+                     hipMemcpyHostToDevice));
+  b->pez_on = true;
   G4_API_END
 }
 

@@ -276,6 +276,8 @@ struct Beam {
   DevBuf<float> xf, yf, pxf, pyf, gf;   // MIXED (gf = gamma - gamma_ref)
   DevBuf<double> theta;                 // both
   DevBuf<double> eloss;                 // per-slice longitudinal field (optional)
+  DevBuf<double> pez;                   // per-particle longitudinal field (optional)
+  bool pez_on = false;
   // scratch of particle passes split over several workgroups per slice
   DevBuf<double2> split_win;
   DevBuf<double> split_red;

@@ -75,6 +75,7 @@ constexpr int kMaxFields = 8;
 struct PushParams {
   double xks, xku, aw, autophase, delz, ax, ay, kx, ky, gradx, grady;
   const double* slice_eloss;  // device [nslice] or null
+  const double* particle_ez;  // device [np] in beam order, or null; replaces slice_eloss
   // mixed-precision phase equation: theta' = -(c0 + eta*(k1 + k2*eta))
   double c0, k1, k2, inv_b0;
   int nfield, pad_;

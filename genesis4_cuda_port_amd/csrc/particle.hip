@@ -472,16 +472,18 @@ __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int 
 
 // RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
 template <int NF>
-__device__ __forceinline__ void push64(const PushParams& p, int slice, double x, double y, double px,
-                                       double py, double& gamma, double& theta, double& base_t,
-                                       double& base_s, double& base_c) {
+__device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t ip, double x,
+                                       double y, double px, double py, double& gamma,
+                                       double& theta, double& base_t, double& base_s,
+                                       double& base_c) {
   const int nf = NF > 0 ? NF : p.nfield;
   double g = gamma;
   double t = theta + p.autophase;
   const double dx = x - p.ax, dy = y - p.ay;
   const double awloc = 1.0 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy;
   const double btpar = 1.0 + px * px + py * py + p.aw * p.aw * awloc * awloc;
-  const double ez = p.slice_eloss ? p.slice_eloss[slice] : 0.0;
+  // load_longitudinal_ez (BeamSolverCUDA.cpp:183-191)
+  const double ez = p.particle_ez ? p.particle_ez[ip] : (p.slice_eloss ? p.slice_eloss[slice] : 0.0);
   double rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
 #pragma unroll
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
@@ -538,9 +540,9 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, double x,
 // Outputs the reduced initial phase of the first field and the FP64 initial
 // phase for the sincos of the final phase.
 template <int NF>
-__device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int slice, float x,
-                                       float y, float px, float py, float& dgamma, double& theta,
-                                       double& base_t, float& base_r) {
+__device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int slice, int64_t ip,
+                                       float x, float y, float px, float py, float& dgamma,
+                                       double& theta, double& base_t, float& base_r) {
   const int nf = NF > 0 ? NF : p.nfield;
   float g = dgamma;
   const double t0 = theta + p.autophase;
@@ -550,7 +552,8 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   const float awloc = 1.0f + awm1;
   const float aw2 = float(p.aw * p.aw);
   const float db0 = px * px + py * py + aw2 * (awm1 * (2.0f + awm1));
-  const float ez = p.slice_eloss ? float(p.slice_eloss[slice]) : 0.0f;
+  const float ez = p.particle_ez ? float(p.particle_ez[ip])
+                                 : (p.slice_eloss ? float(p.slice_eloss[slice]) : 0.0f);
   float rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
 #pragma unroll
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
@@ -732,12 +735,12 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
   if (st & ST_PUSH) {
     if constexpr (sizeof(P) == 8) {
       double gg = g;
-      push64<NF>(prm.push, s, x, y, px, py, gg, th, base_t, base_s, base_c);
+      push64<NF>(prm.push, s, i, x, y, px, py, gg, th, base_t, base_s, base_c);
       g = gg;
       have_base = prm.push.f[0].rharm == 1.0;
     } else {
       float gg = g;
-      push32<NF>(prm.push, mk, s, x, y, px, py, gg, th, base_t, base_r);
+      push32<NF>(prm.push, mk, s, i, x, y, px, py, gg, th, base_t, base_r);
       g = gg;
       have_base = prm.push.f[0].rharm == 1.0;
     }

@@ -50,6 +50,7 @@ struct g4h_run {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int field_generation = 0;  // of the field when captured
+  const double* pez = nullptr;  // per-particle longitudinal field read by the graph
   Context::ParamArena arena;
 
   ~g4h_run() {
@@ -87,6 +88,7 @@ struct g4h_run {
     p.t2 = make_trans(st, g0, delz * 0.5, true);
     Field* fl[1] = {f};
     make_push(p.push, st, g0, delz, fl, 1, nullptr);
+    p.push.particle_ez = b->pez_on ? b->pez.get() : nullptr;
     p.r56 = make_r56(st, g0, cfg.lambda0);
     if (p.r56 != 0.0) p.stages |= ST_R56;
     const bool src_on = make_dep(p.dep, f, b, st, delz) && b->np > 0;
@@ -219,6 +221,7 @@ g4h_status g4h_run_capture(g4h_run* r) {
   G4_HIP(hipStreamEndCapture(ctx->stream, &r->graph));
   G4_HIP(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
   r->field_generation = r->f->generation;
+  r->pez = r->b->pez_on ? r->b->pez.get() : nullptr;
   r->done = static_cast<int>(r->steps.size());
   require(r->nrec == r->end_nrec && r->f->first == r->end_first,
           "captured run differs from the eager run");
@@ -234,6 +237,8 @@ g4h_status g4h_run_replay(g4h_run* r) {
   require(r->f->first == r->first0,
           "re-initialise the field (upload or fill with the run's initial ring offset) before "
           "replaying");
+  require((r->b->pez_on ? r->b->pez.get() : nullptr) == r->pez,
+          "the beam's per-particle longitudinal field was set or cleared after the capture");
   r->ctx->activate();
   r->f->accuslip = r->acc0;
   G4_HIP(hipGraphLaunch(r->exec, r->ctx->stream));

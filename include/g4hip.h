@@ -162,6 +162,17 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* beam, g4h_field* const* 
                             int nfield, const g4h_step* und, double delz,
                             const double* slice_eloss);
 
+/* Per-particle longitudinal field of the push, the UseParticleEz kernels of
+ * BeamSolverCUDA::advance (src/Core/BeamSolverCUDA.cpp:183-191, uploaded
+ * there at :523-547 as efield.getEField(ip) - longESC[is]/511000).  ez: host,
+ * beam->np values in beam order (slice offsets as at creation), in the units
+ * of slice_eloss (subtracted from dgamma/dz); copied to the device and used
+ * by every later g4h_beam_advance and g4h_run_advance of this beam in place
+ * of slice_eloss (which g4h_beam_advance then rejects) until cleared with
+ * ez = NULL.  Set it before g4h_run_capture to have a graph read it (new
+ * values set later are read by the replays; clearing it is an error there). */
+g4h_status g4h_beam_set_particle_ez(g4h_beam* beam, const double* ez);
+
 /* Field::track = solver->init + solver->advance (src/Core/Field.cpp:204-212;
  * FieldSolverADICUDA::advance :638-818, FieldSolverFFTCUDA::advance
  * :325-483): source deposition from the beam, then one ADI or FFT step. */

@@ -363,11 +363,14 @@ void orc_apply_r56(int64_t n, const double* gamma, double* theta, const g4h_step
 }
 
 // BeamSolverCUDA::advance (BeamSolverCUDA.cpp:449-683) with the RK4
-// two-register scheme of push_longitudinal_particles_* (:225-409).
-void orc_beam_advance(int nslice, const int32_t* off, const double* x, const double* y,
-                      const double* px, const double* py, double* gamma, double* theta,
-                      const orc_field* fields, int nfield, const g4h_step* st,
-                      double gamma0, double delz, const double* slice_eloss) {
+// two-register scheme of push_longitudinal_particles_* (:225-409).  The
+// longitudinal field of a particle is particle_ez[ip] when given (the
+// UseParticleEz kernels, :183-191 and :523-547), else slice_eloss[is].
+static void beam_advance(int nslice, const int32_t* off, const double* x, const double* y,
+                         const double* px, const double* py, double* gamma, double* theta,
+                         const orc_field* fields, int nfield, const g4h_step* st,
+                         double gamma0, double delz, const double* slice_eloss,
+                         const double* particle_ez) {
   if (nslice <= 0 || nfield <= 0 || nfield > 8) return;
   FieldView fv[8];
   double rharm[8];
@@ -388,8 +391,9 @@ void orc_beam_advance(int nslice, const int32_t* off, const double* x, const dou
 
 #pragma omp parallel for schedule(dynamic, 4)
   for (int is = 0; is < nslice; ++is) {
-    const double ez = slice_eloss ? slice_eloss[is] : 0.0;  // :555
+    const double ez_slice = slice_eloss ? slice_eloss[is] : 0.0;  // :555
     for (int64_t ip = off[is]; ip < off[is + 1]; ++ip) {
+      const double ez = particle_ez ? particle_ez[ip] : ez_slice;
       double g = gamma[ip];
       double t = theta[ip] + autophase;
       const double xp = x[ip], yp = y[ip], pxp = px[ip], pyp = py[ip];
@@ -431,6 +435,22 @@ void orc_beam_advance(int nslice, const int32_t* off, const double* x, const dou
       theta[ip] = t;
     }
   }
+}
+
+void orc_beam_advance(int nslice, const int32_t* off, const double* x, const double* y,
+                      const double* px, const double* py, double* gamma, double* theta,
+                      const orc_field* fields, int nfield, const g4h_step* st,
+                      double gamma0, double delz, const double* slice_eloss) {
+  beam_advance(nslice, off, x, y, px, py, gamma, theta, fields, nfield, st, gamma0, delz,
+               slice_eloss, nullptr);
+}
+
+void orc_beam_advance_ez(int nslice, const int32_t* off, const double* x, const double* y,
+                         const double* px, const double* py, double* gamma, double* theta,
+                         const orc_field* fields, int nfield, const g4h_step* st,
+                         double gamma0, double delz, const double* particle_ez) {
+  beam_advance(nslice, off, x, y, px, py, gamma, theta, fields, nfield, st, gamma0, delz,
+               nullptr, particle_ez);
 }
 
 void orc_cell_index(int64_t n, const double* x, const double* y, double gridmax,

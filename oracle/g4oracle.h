@@ -18,6 +18,9 @@
  *   orc_apply_r56          small-angle R56 = 2 theta^2 (ld + 2 lb / 3)
  *   orc_beam_advance       zero-field closed form; resonant particle at rest;
  *                          4th-order convergence of RK4 in a constant field
+ *   orc_beam_advance_ez    zero-field closed form per particle (gamma falls
+ *                          by ez[i] delz); equals orc_beam_advance when
+ *                          ez[i] is its slice's loss
  *   orc_deposit            single-particle known answers; partition of unity
  *   orc_cell_index         brute-force floor rule incl. grid-edge inputs
  *   orc_adi_*              Thomas vs dense complex solve (numpy); factors
@@ -82,6 +85,13 @@ void orc_beam_advance(int nslice, const int32_t* off, const double* x, const dou
                       const double* px, const double* py, double* gamma, double* theta,
                       const orc_field* fields, int nfield, const g4h_step* st,
                       double gamma0, double delz, const double* slice_eloss);
+
+/* The same with a longitudinal field per particle (beam order), the
+ * UseParticleEz kernels (BeamSolverCUDA.cpp:183-191,523-547). */
+void orc_beam_advance_ez(int nslice, const int32_t* off, const double* x, const double* y,
+                         const double* px, const double* py, double* gamma, double* theta,
+                         const orc_field* fields, int nfield, const g4h_step* st,
+                         double gamma0, double delz, const double* particle_ez);
 
 /* Source construction (FieldSolverADICUDA.cpp:75-174, FFT twin
  * FieldSolverFFTCUDA.cpp:77-155,383-395).  Writes full source planes in

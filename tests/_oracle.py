@@ -39,6 +39,8 @@ PROTO = {
     "orc_apply_r56": (None, [C.c_int64, _dp, _dp, _sp, C.c_double, C.c_double]),
     "orc_beam_advance": (None, [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _fp, C.c_int, _sp,
                                 C.c_double, C.c_double, _dp]),
+    "orc_beam_advance_ez": (None, [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _fp, C.c_int,
+                                   _sp, C.c_double, C.c_double, _dp]),
     "orc_deposit": (C.c_int, [C.c_int, _ip, _dp, _dp, _dp, _dp, _fp, _sp, _dp, C.c_double, _dp,
                               _dp]),
     "orc_cell_index": (None, [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_int, _ip]),

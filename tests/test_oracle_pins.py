@@ -172,6 +172,43 @@ def test_push_zero_field_closed_form():
     assert np.allclose(o.theta, b.theta + 0.25 + dth, rtol=0, atol=1e-7)
 
 
+def _push_ez(c, field_re, field_im, st, dz, pez):
+    o = c.beam.copy()
+    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, field_re, field_im)
+    arr = (_oracle.OrcField * 1)(of.s)
+    arr[0].re = dp(of.re)
+    arr[0].im = dp(of.im)
+    L.orc_beam_advance_ez(c.nslice, ip(o.offsets), dp(o.x), dp(o.y), dp(o.px), dp(o.py),
+                          dp(o.gamma), dp(o.theta), arr, 1, st, GAMMA0, dz,
+                          dp(np.ascontiguousarray(pez)))
+    return o
+
+
+def test_push_particle_ez_zero_field_closed_form():
+    """Without a field gamma' = -ez exactly, so every particle's energy falls
+    by its own ez * delz: a per-particle field read at a wrong index or with
+    a wrong sign fails particle by particle."""
+    c = cases.make_case(nslice=3, npart=48, ngrid=9, shotnoise=False)
+    st = quad_step()
+    z = np.zeros(c.nslice * c.ngrid ** 2)
+    rng = np.random.default_rng(5)
+    pez = rng.uniform(-3e-3, 3e-3, c.beam.gamma.size)
+    o = _push_ez(c, z, z, st, st.delz, pez)
+    assert np.allclose(o.gamma, c.beam.gamma - pez * st.delz, rtol=0, atol=1e-10)
+
+
+def test_push_particle_ez_reduces_to_slice_loss():
+    """A per-particle field equal to its slice's loss is the slice form, bit
+    for bit, with the radiation field on."""
+    c = cases.make_case(nslice=3, npart=64, ngrid=9, seed_power=1e6)
+    st = quad_step()
+    el = np.array([2e-3, -1e-3, 5e-4])
+    npart = np.diff(c.beam.offsets)
+    a = _push(c, c.seed_re, c.seed_im, st, st.delz, el)
+    b = _push_ez(c, c.seed_re, c.seed_im, st, st.delz, np.repeat(el, npart))
+    assert np.array_equal(a.gamma, b.gamma) and np.array_equal(a.theta, b.theta)
+
+
 def test_push_resonant_particle_is_stationary():
     """An on-axis particle at the resonant energy keeps its phase: theta' ~ 0."""
     c = cases.make_case(nslice=1, npart=4, ngrid=9, shotnoise=False)
