@@ -157,6 +157,7 @@ PROTOTYPES = {
     "g4h_beam_advance": (C.c_int, [_vp, _vp, C.POINTER(_vp), C.c_int, C.POINTER(Step),
                                    C.c_double, _dp]),
     "g4h_beam_set_particle_ez": (C.c_int, [_vp, _dp]),
+    "g4h_beam_add_energy": (C.c_int, [_vp, _vp, _dp]),
     "g4h_field_track": (C.c_int, [_vp, _vp, _vp, C.POINTER(Step), C.c_double]),
     "g4h_apply_slippage": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      C.POINTER(C.c_int)]),

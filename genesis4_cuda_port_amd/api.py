@@ -161,6 +161,10 @@ class Beam:
         el = None if slice_eloss is None else _dp(_f64(slice_eloss, self.nslice))
         check(lib().g4h_beam_advance(self.ctx.h, self.h, arr, len(fields), C.byref(step), delz, el))
 
+    def add_energy(self, dgamma):
+        """gamma += dgamma[slice] on the device (nslice values)."""
+        check(lib().g4h_beam_add_energy(self.ctx.h, self.h, _dp(_f64(dgamma, self.nslice))))
+
     def set_particle_ez(self, ez=None):
         """Per-particle longitudinal field of later pushes (beam order, np
         values, units of slice_eloss); None clears it."""

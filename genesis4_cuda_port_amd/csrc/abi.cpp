@@ -789,6 +789,20 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* b, g4h_field* const* fie
   G4_API_END
 }
 
+g4h_status g4h_beam_add_energy(g4h_context* ctx, g4h_beam* b, const double* dgamma) {
+  G4_API_BEGIN
+  require(ctx && dgamma, "null argument");
+  check_beam(b);
+  ctx->activate();
+  DevBuf<double>& kb = b->kick;
+  // the buffer may still be read by a kick queued earlier
+  G4_HIP(hipStreamSynchronize(ctx->stream));
+  if (kb.size() != static_cast<size_t>(b->nslice)) kb.alloc(ctx, b->nslice);
+  G4_HIP(hipMemcpy(kb.get(), dgamma, sizeof(double) * b->nslice, hipMemcpyHostToDevice));
+  launch_slice_kick(ctx, b, kb.get());
+  G4_API_END
+}
+
 g4h_status g4h_beam_set_particle_ez(g4h_beam* b, const double* ez) {
   G4_API_BEGIN
   check_beam(b);

@@ -277,6 +277,7 @@ struct Beam {
   DevBuf<double> theta;                 // both
   DevBuf<double> eloss;                 // per-slice longitudinal field (optional)
   DevBuf<double> pez;                   // per-particle longitudinal field (optional)
+  DevBuf<double> kick;                  // per-slice energy change (g4h_beam_add_energy)
   bool pez_on = false;
   // scratch of particle passes split over several workgroups per slice
   DevBuf<double2> split_win;

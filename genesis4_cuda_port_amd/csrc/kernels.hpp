@@ -116,6 +116,8 @@ struct ParticleParams {
 void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
                      const ParticleParams* d_table = nullptr);
 void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell);
+// gamma += d_dgamma[slice] (device, nslice values) for every particle
+void launch_slice_kick(Context* ctx, Beam* b, const double* d_dgamma);
 
 // Host construction of parameter blocks from a lattice step.
 TransParams make_trans(const g4h_step& st, double gamma0, double delz, bool last);

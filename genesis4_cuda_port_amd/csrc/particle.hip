@@ -1258,6 +1258,15 @@ __global__ void cell_index_kernel(BeamPtr<P> b, int64_t n, double gm, double dg,
                 : -1;
 }
 
+// gamma += dgamma[slice] for every particle of the slice (one workgroup
+// per slice; MIXED adds to gamma - gamma_ref).
+template <class P>
+__global__ void slice_kick_kernel(P* g, const int32_t* off, const double* dgamma) {
+  const int s = blockIdx.x;
+  const P d = static_cast<P>(dgamma[s]);
+  for (int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) g[i] += d;
+}
+
 // Zero the cells the previous deposition left in each slice's source plane
 // (the source rectangle) before a split pass deposits into it.
 template <class P, bool TAB>
@@ -1377,6 +1386,18 @@ void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell) {
       hipLaunchKernelGGL(cell_index_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream, bp,
                          b->np, f->gridmax, f->dgrid, f->ngrid, d_cell);
     }
+  });
+}
+
+void launch_slice_kick(Context* ctx, Beam* b, const double* d_dgamma) {
+  if (b->nslice <= 0 || b->np <= 0) return;
+  ctx->launch(G4H_K_OTHER, [&] {
+    if (b->prec == G4H_FP64)
+      hipLaunchKernelGGL(slice_kick_kernel<double>, dim3(b->nslice), dim3(256), 0, ctx->stream,
+                         b->gd.get(), b->off.get(), d_dgamma);
+    else
+      hipLaunchKernelGGL(slice_kick_kernel<float>, dim3(b->nslice), dim3(256), 0, ctx->stream,
+                         b->gf.get(), b->off.get(), d_dgamma);
   });
 }
 

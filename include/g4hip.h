@@ -173,6 +173,15 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* beam, g4h_field* const* 
  * values set later are read by the replays; clearing it is an error there). */
 g4h_status g4h_beam_set_particle_ez(g4h_beam* beam, const double* ez);
 
+/* Energy change per slice applied to the device beam: gamma += dgamma[is]
+ * for every particle of slice is (host, nslice values).  Beam::track applies
+ * the collective effects (wakes, Collective::apply) and incoherent radiation
+ * (Incoherent::apply) between the push and the R56 (src/Core/Beam.cpp:
+ * 127-128), on the host particle arrays, which the CUDA path no longer
+ * keeps current; with this call the energy change they compute reaches the
+ * device state instead.  Enqueued on the context stream. */
+g4h_status g4h_beam_add_energy(g4h_context* ctx, g4h_beam* beam, const double* dgamma);
+
 /* Field::track = solver->init + solver->advance (src/Core/Field.cpp:204-212;
  * FieldSolverADICUDA::advance :638-818, FieldSolverFFTCUDA::advance
  * :325-483): source deposition from the beam, then one ADI or FFT step. */

@@ -1,6 +1,7 @@
 """GPU: the per-particle longitudinal field of the push (the UseParticleEz
 kernels of BeamSolverCUDA.cpp:183-191,523-547) against the FP64 oracle, in
-the single push and in the fused run."""
+the single push and in the fused run; the per-slice energy change of the
+collective effects (Beam.cpp:127-128) applied to the device beam."""
 from __future__ import annotations
 
 import numpy as np
@@ -97,3 +98,28 @@ def test_run_particle_ez_zero_field(gpu_ctx, prec):
     err = np.abs(g - want).max()
     print(f"  max |dgamma error| {err:.3e}", flush=True)
     assert err < (1e-9 if prec == api.FP64 else 2e-3)
+
+
+@pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
+def test_add_energy_per_slice(gpu_ctx, prec):
+    """Every particle of slice s gains dgamma[s]; nothing else moves.
+    Includes an empty slice."""
+    print(f"case add_energy prec={prec}", flush=True)
+    c = cases.make_case(nslice=5, npart=256, ngrid=17)
+    off = c.beam.offsets.copy()
+    off[2] = off[3]  # the third slice is empty, the second takes its particles
+    dg = np.array([1e-3, -2.5e-3, 7.0, 0.0, 0.125])
+    b = api.Beam(gpu_ctx, prec, off, np.full(c.nslice, 1000.0), GAMMA0)
+    b.upload(c.beam.x, c.beam.y, c.beam.px, c.beam.py, c.beam.gamma, c.beam.theta)
+    before = b.download()
+    b.add_energy(dg)
+    after = b.download()
+    want = np.repeat(dg, np.diff(off))
+    d = after[4] - before[4]
+    if prec == api.FP64:
+        assert np.array_equal(after[4], before[4] + want)
+    else:
+        assert np.abs(d - want).max() < 4e-6
+    for k in (0, 1, 2, 3, 5):
+        assert np.array_equal(after[k], before[k])
+    b.close()
