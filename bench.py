@@ -136,12 +136,14 @@ def power_bunching(braw, fraw, current, ngrid, gridmax):
     return fq[:, 0], bq[:, 25]
 
 
-def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int):
+def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int,
+                 threads: int = 0):
     """The FP64 oracle on the host cores over the first nslice_cpu slices of the
     same workload (one untimed step, then nsteps_cpu timed steps)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     if threads <= 0:
         threads = min(16, len(os.sched_getaffinity(0)))
     _oracle.lib().orc_set_threads(threads)
@@ -394,6 +396,10 @@ def main() -> int:
         log("cpu baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
                                            a.cpu_steps)
+        # and one host thread on a smaller sample (the scalar oracle)
+        one = cpu_baseline(beam, steps, a.ngrid, gridmax, max(1, a.cpu_slices // 32),
+                           a.cpu_steps, threads=1)
+        out["cpu_baseline"]["single_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
     if not a.no_adi:
         log("time-dependent ADI run (FP64)")
         out["time_dependent_adi_fp64"] = time_dependent_adi(a.steps, a.warmup)
