@@ -321,7 +321,7 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
 template <class P, bool EXACT = false>
 __device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, P idg, int ng, int& ix, int& iy,
                                         P& wx, P& wy) {
-  if ((x <= -gm) || (x >= gm) || (y <= -gm) || (y >= gm)) return false;
+  if (!(x > -gm && x < gm && y > -gm && y < gm)) return false;  // NaN: off the grid
   if constexpr (sizeof(P) == 8) {
     const P wxr = EXACT ? (x + gm) / dg : (x + gm) * idg;
     const P wyr = EXACT ? (y + gm) / dg : (y + gm) * idg;

@@ -127,7 +127,7 @@ def test_beam_advance_fp64(gpu_ctx, nfield, eloss):
 
 def test_cell_index_bitexact(gpu_ctx):
     """Indices of random and adversarial positions (grid nodes, +-gridmax,
-    one ulp either side) must equal the oracle's exactly."""
+    one ulp either side, NaN) must equal the oracle's exactly."""
     c = cases.make_case(nslice=2, npart=4096, ngrid=31)
     rng = np.random.default_rng(3)
     n = c.beam.x.size
@@ -137,7 +137,7 @@ def test_cell_index_bitexact(gpu_ctx):
     x = np.where(rng.random(n) < 0.5, rng.uniform(-1.1 * gm, 1.1 * gm, n), nodes)
     y = np.where(rng.random(n) < 0.5, rng.uniform(-1.1 * gm, 1.1 * gm, n), nodes[::-1])
     special = np.array([gm, -gm, np.nextafter(gm, 0), np.nextafter(-gm, 0), 0.0,
-                        np.nextafter(gm - dg, gm), gm - dg, np.nextafter(gm - dg, 0)])
+                        np.nextafter(gm - dg, gm), gm - dg, np.nextafter(gm - dg, 0), np.nan])
     x[:special.size] = special
     y[:special.size] = special[::-1]
     bm = c.beam.copy()
