@@ -308,7 +308,18 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
 // the per-block partial sums reduced in block order, bit-identical stats.
 
 constexpr int kLane = 64;
-constexpr int kCh = 8;  // elements per lane per chunk
+
+// c, cbet, cwet of the step into LDS ([3][G]) by the one-wave workgroup
+__device__ __forceinline__ void stage_factors(double2* coef, const double2* c_d,
+                                              const double2* cbet_d, const double2* cwet_d,
+                                              int G) {
+  for (int k = threadIdx.x; k < G; k += blockDim.x) {
+    coef[k] = c_d[k];
+    coef[G + k] = cbet_d[k];
+    coef[2 * G + k] = cwet_d[k];
+  }
+  __syncthreads();
+}
 
 // lane i <- lane i-1 (wave_shr:1); lane 0 keeps `edge`
 __device__ __forceinline__ double2 from_lane_below(double2 v, double2 edge) {
@@ -333,8 +344,9 @@ __device__ __forceinline__ double2 from_lane_above(double2 v, double2 edge) {
   return o;
 }
 
-// x sweep: lane t of block b owns row iy = 64 b + t and walks its columns.
-template <class P>
+// x sweep: lane t of block b owns row iy = 64 b + t and walks its columns,
+// kCh elements per lane per chunk.
+template <class P, int kCh>
 __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u, double2* ys,
                                                            int nslice, int G, double cs,
                                                            const double2* __restrict__ c_d,
@@ -346,6 +358,13 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
   using T2 = typename A2<P>::T;
   const int p = blockIdx.x;
   const int t = threadIdx.x;
+  // the Thomas factors in LDS: read in the recurrence at one element per
+  // step, from memory they would put a load latency on every step
+  extern __shared__ double2 coef[];
+  stage_factors(coef, c_d, cbet_d, cwet_d, G);
+  const double2* cl = coef;
+  const double2* cb = coef + G;
+  const double2* cw = coef + 2 * G;
   const int iy = blockIdx.y * kLane + t;
   const bool act = iy < G;
   // the row beyond the block, loaded by the wave's edge lanes
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
         const double2 below = from_lane_above(cen[c], edg[c]);  // row iy + 1
         double2 r = rhs_of(cen[c], below, above, cs);
         if (srow && k >= rc.x && k < rc.z) r = zadd(r, sv[c]);
-        yprev = zmul(zsub(r, zmul(c_d[k], yprev)), cbet_d[k]);
+        yprev = zmul(zsub(r, zmul(cl[k], yprev)), cb[k]);
         if (act) yrow[k] = yprev;
       }
     }
@@ -431,7 +450,7 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
     for (int c = kCh - 1; c >= 0; --c) {
       const int k = k0 + c;
       if (k < G) {
-        const double2 uu = (k == G - 1) ? y[c] : zsub(y[c], zmul(cwet_d[k + 1], unext));
+        const double2 uu = (k == G - 1) ? y[c] : zsub(y[c], zmul(cw[k + 1], unext));
         unext = uu;
         if (act) {
           T2 o;
@@ -448,7 +467,7 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
 // the backward pass writes the field and the block's partial field sums
 // (partials[(p * nblk + b) * 8 + k], centre cell at the tail), reduced in
 // block order afterwards.
-template <class P>
+template <class P, int kCh>
 __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u, double2* ys,
                                                            int nslice, int G, double cs,
                                                            const double2* __restrict__ c_d,
@@ -461,6 +480,11 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
   const int p = blockIdx.x;
   const int t = threadIdx.x;
   const int nblk = gridDim.y;
+  extern __shared__ double2 coef[];
+  stage_factors(coef, c_d, cbet_d, cwet_d, G);
+  const double2* cl = coef;
+  const double2* cb = coef + G;
+  const double2* cw = coef + 2 * G;
   const int ix = blockIdx.y * kLane + t;
   const bool act = ix < G;
   const int ex = t == 0 ? ix - 1 : (t == kLane - 1 ? ix + 1 : -1);
@@ -516,7 +540,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
         const double2 right = from_lane_above(v[r], e[r]);  // column ix + 1
         double2 rr = rhs_of(v[r], right, left, cs);
         if (scol && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
-        yprev = zmul(zsub(rr, zmul(c_d[k], yprev)), cbet_d[k]);
+        yprev = zmul(zsub(rr, zmul(cl[k], yprev)), cb[k]);
         if (act) st_stream(&yp[static_cast<int64_t>(k) * G + cx], yprev);
       }
     }
@@ -544,7 +568,7 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
     for (int r = kCh - 1; r >= 0; --r) {
       const int k = k0 + r;
       if (act && k < G) {
-        const double2 uu = (k == G - 1) ? y[r] : zsub(y[r], zmul(cwet_d[k + 1], unext));
+        const double2 uu = (k == G - 1) ? y[r] : zsub(y[r], zmul(cw[k + 1], unext));
         unext = uu;
         T2 o;
         o.x = static_cast<P>(uu.x);
@@ -595,11 +619,14 @@ void adi_step(Field* f, bool source_on, bool stats) {
   const size_t ldsy = sizeof(double2) * (3 * static_cast<size_t>(G) + 2 * static_cast<size_t>(G + 2));
   const int add = source_on ? 1 : 0;
   // variant 0: workgroup per plane (its x sweep stages G + 2 rows of kB
-  // columns plus three coefficient tables in LDS, so G <= 360); variant 1:
-  // wave per 64 lines, any G.  Tuned per shape.
-  const int nvar = G <= 360 ? 2 : 1;
-  TuneScope tune(f->ctx, TUNE_ADI, {G, f->nslice, static_cast<int>(f->prec)}, nvar);
-  if (nvar == 1 || tune.variant() == 1) {
+  // columns plus three coefficient tables in LDS, so G <= 360; above that
+  // it runs as variant 1); variants 1 and 2: wave per 64 lines, any G, with
+  // the loads 8 or 16 elements ahead of the recurrence (16 holds more loads
+  // in flight, which is what a sweep over few planes waits on, for more
+  // registers).  Tuned per shape; all three give bit-identical fields.
+  TuneScope tune(f->ctx, TUNE_ADI, {G, f->nslice, static_cast<int>(f->prec)}, 3);
+  const int variant = (tune.variant() == 0 && G > 360) ? 1 : tune.variant();
+  if (variant >= 1) {
     const int nblk = (G + kLane - 1) / kLane;
     double* partials = nullptr;
     if (stats) {
@@ -607,17 +634,25 @@ void adi_step(Field* f, bool source_on, bool stats) {
       if (f->col_partials.size() < need) f->col_partials.alloc(f->ctx, need);
       partials = f->col_partials.get();
     }
-    f->ctx->launch(G4H_K_ADI_X, [&] {
-      hipLaunchKernelGGL(adi_x_lane_kernel<P>, dim3(f->nslice, nblk), dim3(kLane), 0,
-                         f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
-                         f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add);
-    });
-    f->ctx->launch(G4H_K_ADI_Y, [&] {
-      hipLaunchKernelGGL(adi_y_lane_kernel<P>, dim3(f->nslice, nblk), dim3(kLane), 0,
-                         f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
-                         f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add,
-                         partials);
-    });
+    const size_t ldsc = sizeof(double2) * 3 * static_cast<size_t>(G);
+    auto sweeps = [&](auto ch) {
+      constexpr int CH = decltype(ch)::value;
+      f->ctx->launch(G4H_K_ADI_X, [&] {
+        hipLaunchKernelGGL((adi_x_lane_kernel<P, CH>), dim3(f->nslice, nblk), dim3(kLane), ldsc,
+                           f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                           f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add);
+      });
+      f->ctx->launch(G4H_K_ADI_Y, [&] {
+        hipLaunchKernelGGL((adi_y_lane_kernel<P, CH>), dim3(f->nslice, nblk), dim3(kLane), ldsc,
+                           f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                           f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add,
+                           partials);
+      });
+    };
+    if (variant == 1)
+      sweeps(std::integral_constant<int, 8>{});
+    else
+      sweeps(std::integral_constant<int, 16>{});
     if (stats) field_reduce_partials(f, partials, nblk);
     return;
   }

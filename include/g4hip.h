@@ -60,7 +60,9 @@ g4h_status g4h_context_memory(const g4h_context* ctx, uint64_t* bytes_in_use);
  * per shape the context tries each on its first launches and keeps the
  * fastest.  No counterpart in the reference (its kernels have one launch
  * shape each).  Families: */
-#define G4H_TUNE_ADI 0   /* 0: workgroup per plane, 1: wave per 64 lines */
+#define G4H_TUNE_ADI 0   /* 0: workgroup per plane (ngrid <= 360, else as 1),
+                            1 / 2: wave per 64 lines, loads 8 / 16 elements
+                            ahead; bit-identical results */
 #define G4H_TUNE_SPLIT 1 /* workgroups per slice of the particle pass (>= 1; by
                             default chosen from the slice count so that few
                             slices still fill the GPU; results agree to
