@@ -137,9 +137,13 @@ def power_bunching(braw, fraw, current, ngrid, gridmax):
 
 
 def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int,
-                 threads: int = 0):
+                 threads: int = 0, keep: bool = False):
     """The FP64 oracle on the host cores over the first nslice_cpu slices of the
-    same workload (one untimed step, then nsteps_cpu timed steps)."""
+    same workload (one untimed step, then nsteps_cpu timed steps).  With keep,
+    also returns the oracle's last beam and field records (after
+    1 + nsteps_cpu steps): slippage carries the field from the tail to the
+    head, so the first slices of the full beam evolve as this sub-beam does,
+    and the GPU runs' records of those slices compare with these."""
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle
     if threads <= 0:
@@ -161,12 +165,27 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int,
     timed = steps_array(st[1:1 + nsteps_cpu])
     sub, _, _ = _oracle.run(sub, of, warm, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
     t0 = time.perf_counter()
-    _oracle.run(sub, of, timed, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
+    _, braw, fraw = _oracle.run(sub, of, timed, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
     dt = time.perf_counter() - t0
-    return {"value": n * nsteps_cpu / dt, "unit": "particle-slice-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle, first {nslice_cpu} of 4096 slices x {npart} particles, "
-                      f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
+    res = {"value": n * nsteps_cpu / dt, "unit": "particle-slice-steps/s", "cores": threads,
+           "kind": "port",
+           "sample": f"FP64 oracle, first {nslice_cpu} of 4096 slices x {npart} particles, "
+                     f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
+    return (res, braw[-1], fraw[-1]) if keep else res
+
+
+def slice_quantities(braw, fraw, current, ngrid, gridmax):
+    """Per-slice power, bunching and energy spread of one record."""
+    from genesis4_cuda_port_amd import api
+    dgrid = 2 * gridmax / (ngrid - 1)
+    fq, _ = api.finalize_field(np.ravel(fraw), dgrid, LAMBDA0, LAMBDA0)
+    bq, _ = api.finalize_beam(np.ravel(braw), 1, current, GAMMA0)
+    return {"power": fq[:, 0], "bunching": bq[:, 25], "energyspread": bq[:, 1]}
+
+
+def rel_errors(got, ref):
+    return {f"{k}_rel": float(np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300))
+            for k in ref}
 
 
 def time_dependent_adi(nsteps: int, warmup: int):
@@ -394,8 +413,21 @@ def main() -> int:
             "at_step": a.warmup + a.steps}
     if not a.no_cpu:
         log("cpu baseline (FP64 oracle)")
-        out["cpu_baseline"] = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices,
-                                           a.cpu_steps)
+        res, ob, of_ = cpu_baseline(beam, steps, a.ngrid, gridmax, a.cpu_slices, a.cpu_steps,
+                                    keep=True)
+        out["cpu_baseline"] = res
+        # the GPU runs against the FP64 CPU oracle on the same slices and step
+        n, k = a.cpu_slices, 1 + a.cpu_steps
+        if k < len(bm):
+            cur = beam.current[:n]
+            ref = slice_quantities(ob, of_, cur, a.ngrid, gridmax)
+            vs = {"slices": n, "at_step": k,
+                  "mixed": rel_errors(slice_quantities(bm[k][:n], fm[k][:n], cur, a.ngrid,
+                                                       gridmax), ref)}
+            if not a.no_fp64:
+                vs["fp64"] = rel_errors(slice_quantities(b64[k][:n], f64[k][:n], cur, a.ngrid,
+                                                         gridmax), ref)
+            out["vs_cpu_oracle"] = vs
         # and one host thread on a smaller sample (the scalar oracle)
         one = cpu_baseline(beam, steps, a.ngrid, gridmax, max(1, a.cpu_slices // 32),
                            a.cpu_steps, threads=1)
