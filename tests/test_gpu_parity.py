@@ -125,6 +125,41 @@ def test_beam_advance_fp64(gpu_ctx, nfield, eloss):
     assert relerr(t, o.theta) < 1e-13
 
 
+@pytest.mark.parametrize("prec,nfield", [(api.FP64, 8), (api.MIXED, 3)])
+def test_beam_advance_many_fields(gpu_ctx, prec, nfield):
+    """The general multi-field push up to its limit of 8 fields (harmonics
+    1..nfield), against the oracle."""
+    print(f"case beam_advance_many prec={prec} nfield={nfield}", flush=True)
+    c = cases.make_case(nslice=4, npart=512, ngrid=33, seed_power=1e7)
+    st = quad_step(autophase=0.1)
+    fields, ofields = [], []
+    for h in range(1, nfield + 1):
+        re, im = noisy_field(c, seed=20 + h, scale=100.0 / h)
+        fields.append(gpu_field(gpu_ctx, c, prec, harm=h, re=re, im=im, first=1))
+        ofields.append(OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, h, re, im,
+                              first=1))
+    b = gpu_beam(gpu_ctx, c, prec)
+    b.advance(fields, st, st.delz)
+    g, t = b.download()[4:6]
+    o = c.beam.copy()
+    arr = (_oracle.OrcField * nfield)(*[f.s for f in ofields])
+    for k, f in enumerate(ofields):
+        arr[k].re = dp(f.re)
+        arr[k].im = dp(f.im)
+    _oracle.lib().orc_beam_advance(c.nslice, ip(o.offsets), dp(o.x), dp(o.y), dp(o.px), dp(o.py),
+                                   dp(o.gamma), dp(o.theta), arr, nfield, st, GAMMA0, st.delz,
+                                   None)
+    eg, et = relerr(g - GAMMA0, o.gamma - GAMMA0), relerr(t, o.theta)
+    print(f"  gamma {eg:.3e} theta {et:.3e}", flush=True)
+    if prec == api.FP64:
+        assert eg < 1e-12 and et < 1e-13
+    else:
+        assert eg < MIXED_TOL and et < 1e-6
+    for f in fields:
+        f.close()
+    b.close()
+
+
 def test_cell_index_bitexact(gpu_ctx):
     """Indices of random and adversarial positions (grid nodes, +-gridmax,
     one ulp either side, NaN) must equal the oracle's exactly."""
@@ -260,14 +295,14 @@ def test_slippage_bitexact(gpu_ctx, slips, periodic):
         assert f.ring()[1] == of.s.accuslip
 
 
+@pytest.mark.parametrize("nharm", [3, 8])
 @pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
-def test_diag_raw(gpu_ctx, prec):
+def test_diag_raw(gpu_ctx, prec, nharm):
     c = cases.make_case(nslice=5, npart=1024, ngrid=33, seed_power=1e6)
     b = gpu_beam(gpu_ctx, c, prec)
     re, im = noisy_field(c, scale=0.01)
     f = gpu_field(gpu_ctx, c, prec, re=re, im=im, first=2)
     of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, re, im, first=2)
-    nharm = 3
     braw = b.diag_raw(nharm)
     o = c.beam
     oraw = np.zeros_like(braw)
