@@ -793,6 +793,7 @@ g4h_status g4h_beam_add_energy(g4h_context* ctx, g4h_beam* b, const double* dgam
   G4_API_BEGIN
   require(ctx && dgamma, "null argument");
   check_beam(b);
+  if (b->nslice == 0 || b->np == 0) return G4H_OK;
   ctx->activate();
   DevBuf<double>& kb = b->kick;
   // the buffer may still be read by a kick queued earlier
