@@ -135,9 +135,12 @@ g4h_status g4h_field_set_enabled(g4h_field* field, int enabled);
 
 /* Field::initSolver (src/Core/Field.cpp:87-110): solver kind, FFT source
  * filter (FieldSolverFFTCUDA::initSourceFilter, FieldSolverFFTCUDA.cpp:
- * 213-221).  The ADI low-memory switch is accepted for deck compatibility;
- * this build never allocates a dense source grid, so both settings run the
- * same kernels. */
+ * 213-221).  The ADI low-memory switch (low_memory_adisolver,
+ * src/Main/Track.cpp:19-20, which deposits into the right-hand sides
+ * instead of a source grid) is accepted for deck compatibility and has no
+ * effect: both settings keep one source plane per slice, of which the
+ * deposition writes and the solvers read only the slice's footprint
+ * rectangle, and give the same results. */
 g4h_status g4h_field_init_solver(g4h_field* field, g4h_solver_kind kind, int source_filter,
                                  double xcut, double ycut, double sigmoid,
                                  int low_memory_adisolver);
