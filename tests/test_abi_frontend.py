@@ -77,6 +77,8 @@ def test_null_and_bad_sizes_return_status(g4lib):
     assert g4lib.g4h_track_transverse(None, None, None, 0.1, 0) == 1
     assert g4lib.g4h_field_track(None, None, None, None, 0.1) == 1
     assert g4lib.g4h_run_advance(None, 1, None) == 1
+    assert g4lib.g4h_beam_set_particle_ez(None, None) == 1
+    assert g4lib.g4h_beam_add_energy(None, None, None) == 1
 
 
 @pytest.mark.parametrize("dims", [1, 3, 6, 8])
