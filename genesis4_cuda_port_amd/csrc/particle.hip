@@ -1162,10 +1162,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
       double2 v = win[k];
       if (K > 1) {
-        // parts added in part order; unrolled so that the loads of several
-        // parts are in flight together (one workgroup combines the slice)
         v = parts[k];
-#pragma unroll 8
         for (int q = 1; q < K; ++q) {
           const double2 a = parts[static_cast<int64_t>(q) * prm.dep.window_cap + k];
           v.x += a.x;
@@ -1223,7 +1220,6 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       if (K == 1) {
         for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
       } else {
-#pragma unroll 8
         for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
       }
       const int dst = tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12);
