@@ -638,3 +638,30 @@ def test_energy_balance_beam_to_field():
     assert abs(e1) < 0.1 and abs(e2) < 0.05
     assert 1.7 < e1 / e2 < 2.3
     assert abs(2 * e2 - e1) < 5e-3
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_head_slices_evolve_alone(solver):
+    """Slippage carries the field from lower to higher slice index and a
+    slice's particles see only their own plane, so the first n slices of a
+    beam evolve exactly as a beam of those n slices alone: bit for bit
+    (this is what lets a sample of the first slices of a large run be
+    compared with the same slices of the full run)."""
+    c = cases.make_case(nslice=8, npart=256, ngrid=17, nsteps=6, seed_power=1e4)
+    ngrid2 = c.ngrid * c.ngrid
+    full = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, c.seed_re, c.seed_im)
+    _, braw, fraw = _oracle.run(c.beam, full, c.steps, gamma0=GAMMA0, lambda0=LAMBDA0,
+                                solver=solver, sample=c.sample, timerun=True)
+    n = 4
+    npart = int(c.beam.offsets[1] - c.beam.offsets[0])
+    sub = c.beam.copy()
+    for k in ("x", "y", "px", "py", "gamma", "theta"):
+        setattr(sub, k, getattr(c.beam, k)[:n * npart].copy())
+    sub.offsets = c.beam.offsets[:n + 1].copy()
+    sub.current = c.beam.current[:n].copy()
+    part = OField(n, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, c.seed_re[:n * ngrid2],
+                  c.seed_im[:n * ngrid2])
+    _, braw2, fraw2 = _oracle.run(sub, part, c.steps, gamma0=GAMMA0, lambda0=LAMBDA0,
+                                  solver=solver, sample=c.sample, timerun=True)
+    assert np.array_equal(braw[:, :n], braw2) and np.array_equal(fraw[:, :n], fraw2)
+    assert np.abs(fraw[-1, :n, 0]).max() > 0  # the field is not trivially zero
