@@ -85,6 +85,7 @@ def build(clean: bool = False, jobs: int | None = None) -> Path:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
     _build_cli()
+    _build_adapter_check()
     return LIB
 
 
@@ -101,6 +102,23 @@ def _build_cli() -> Path:
         raise RuntimeError(f"g4run link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, CLI)
     return CLI
+
+
+ADAPTER_CHECK = OUT / "g4adapter_check"
+
+
+def _build_adapter_check() -> Path:
+    """A small program that runs a case through the C++ adapter classes of
+    csrc/genesis_gpu.hpp (the reference-shaped layer), linked against
+    libg4hip.so beside it."""
+    tmp = ADAPTER_CHECK.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", str(CSRC / "adapter_check_main.cpp"), "-o",
+           str(tmp), f"-L{OUT}", "-lg4hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"adapter check link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, ADAPTER_CHECK)
+    return ADAPTER_CHECK
 
 
 def build_oracle() -> Path:

@@ -214,6 +214,9 @@ class FieldSolverGPU : public FieldSolver {
     filter_ = filter;
     configured_ = nullptr;
   }
+  // set the field's device solver now (Field::initSolver), so that the fused
+  // loop of Gencore::run, which never calls advance(), runs this solver
+  void bind(Field* f) { configure(f); }
 
  private:
   void configure(Field* f) {
@@ -242,12 +245,16 @@ inline void Field::initSolver(bool isFFT, bool filter, double xc, double yc, dou
                               bool use_gpu, bool low_memory_adisolver) {
   if (!use_gpu) throw Error(G4H_ERR_UNSUPPORTED, "use_gpu=false: use the upstream CPU solvers");
   delete solver_;
+  solver_ = nullptr;
+  FieldSolverGPU* s = nullptr;
   if (isFFT) {
-    solver_ = new FieldSolverFFTGPU();
-    solver_->initSourceFilter(xc, yc, sig, filter);
+    s = new FieldSolverFFTGPU();
+    s->initSourceFilter(xc, yc, sig, filter);
   } else {
-    solver_ = new FieldSolverADIGPU(low_memory_adisolver);
+    s = new FieldSolverADIGPU(low_memory_adisolver);
   }
+  solver_ = s;
+  s->bind(this);
 }
 
 // TrackBeamCUDA (TrackBeamCUDA.cpp:156-278)
@@ -264,13 +271,22 @@ class TrackBeamGPU {
 // BeamSolverCUDA (BeamSolverCUDA.cpp:449-683)
 class BeamSolverGPU {
  public:
+  // slice_eloss: -longESC/511000 per slice (:555); particle_ez: the
+  // short-range field per particle plus that loss, beam order (:523-547),
+  // which replaces it when given
   void advance(double delz, Beam* beam, std::vector<Field*>* fields, Undulator* und,
-               const double* slice_eloss = nullptr) {
+               const double* slice_eloss = nullptr, const double* particle_ez = nullptr) {
     std::vector<g4h_field*> fl;
     for (Field* f : *fields)
       if (f) fl.push_back(f->soa());
+    check(g4h_beam_set_particle_ez(beam->soa(), particle_ez));
     check(g4h_beam_advance(beam->ctx().get(), beam->soa(), fl.data(), static_cast<int>(fl.size()),
-                           &und->step(), delz, slice_eloss));
+                           &und->step(), delz, particle_ez ? nullptr : slice_eloss));
+  }
+  // the energy change per slice of Collective::apply / Incoherent::apply
+  // (Beam.cpp:127-128), applied to the device beam
+  void applyEnergyChange(Beam* beam, const double* dgamma) {
+    check(g4h_beam_add_energy(beam->ctx().get(), beam->soa(), dgamma));
   }
   void track(double delz, Beam* beam, Undulator* und, bool last) { tracker_.track(delz, beam, und, last); }
   void applyR56(Beam* beam, Undulator* und, double reflen) { tracker_.applyR56(beam, und, reflen); }
@@ -340,17 +356,17 @@ class DiagnosticGPU {
   int nharm_ = 1;
 };
 
-// Gencore::run (Gencore.cpp:161-254): with one field the whole loop runs as
-// the fused device loop (g4h_run); with several fields it is driven operator
-// by operator exactly in the reference's order.
+// Gencore::run (Gencore.cpp:161-254): with one field (and `fused`) the whole
+// loop runs as the fused device loop (g4h_run); otherwise it is driven
+// operator by operator exactly in the reference's order.
 class Gencore {
  public:
   static std::vector<DiagnosticRecord> run(Beam* beam, std::vector<Field*>* fields,
                                            Undulator* und, bool isTime, bool periodic,
-                                           int nharm = 1) {
+                                           int nharm = 1, bool fused = true) {
     std::vector<DiagnosticRecord> hist;
     const double sample = beam->slicelength / beam->reflength;
-    if (fields->size() == 1) {
+    if (fused && fields->size() == 1) {
       Field* f = (*fields)[0];
       g4h_run_config cfg{beam->reflength, sample, isTime ? 1 : 0, periodic ? 1 : 0, nharm, 1};
       g4h_run* r = nullptr;

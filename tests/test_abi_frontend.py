@@ -159,3 +159,19 @@ def test_server_and_batch_need_a_device(g4lib):
     assert g4lib.g4h_server_create(0, 1 << 30, 2, C.byref(h)) == 6  # G4H_ERR_NO_DEVICE
     assert g4lib.g4h_server_create(0, 1 << 30, 0, C.byref(h)) == 1  # bad lane count
     assert g4lib.g4h_batch_create(None, 0, 0, 1, None, None, 1, 1e-9, 1, C.byref(h)) == 1
+
+
+def test_adapter_layer_compiles_and_links():
+    """The reference-shaped C++ layer (csrc/genesis_gpu.hpp) is compiled into
+    _lib/g4adapter_check by the build; without a GPU it reports that as an
+    error status, not a crash."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(api.__file__).resolve().parent / "_lib" / "g4adapter_check"
+    assert exe.exists()
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2 and "usage" in res.stderr
+    if api.device_count() == 0:
+        res = subprocess.run([str(exe), "fp64", "adi", "1", "1"], capture_output=True, text=True,
+                             timeout=60)
+        assert res.returncode == 1 and "no GPU" in res.stderr
