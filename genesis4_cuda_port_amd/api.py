@@ -161,6 +161,18 @@ class Beam:
         el = None if slice_eloss is None else _dp(_f64(slice_eloss, self.nslice))
         check(lib().g4h_beam_advance(self.ctx.h, self.h, arr, len(fields), C.byref(step), delz, el))
 
+    def set_current(self, current):
+        """Per-slice current (A), as the source scale and the diagnostics read it."""
+        cur = _f64(current, self.nslice)
+        check(lib().g4h_beam_set_current(self.h, _dp(cur)))
+        self.current = cur
+
+    def info(self):
+        """(nslice, total particles, precision) as the library holds them."""
+        ns, npart, prec = C.c_int(), C.c_int64(), C.c_int()
+        check(lib().g4h_beam_info(self.h, C.byref(ns), C.byref(npart), C.byref(prec)))
+        return int(ns.value), int(npart.value), int(prec.value)
+
     def add_energy(self, dgamma):
         """gamma += dgamma[slice] on the device (nslice values)."""
         check(lib().g4h_beam_add_energy(self.ctx.h, self.h, _dp(_f64(dgamma, self.nslice))))

@@ -95,7 +95,11 @@ g4h_status g4h_beam_upload(g4h_beam* beam, const double* x, const double* y,
 /* Blocking copy back to host FP64 arrays (Beam::unpack_soa_to_beam). */
 g4h_status g4h_beam_download(g4h_beam* beam, double* x, double* y, double* px,
                              double* py, double* gamma, double* theta);
+/* Beam::current per slice (A; include/Beam.h:69, recomputed from the
+ * particle count after a sort at src/Core/Beam.cpp:100-104): the source
+ * scale (FieldSolverADICUDA.cpp:75-96) and the diagnostics read it. */
 g4h_status g4h_beam_set_current(g4h_beam* beam, const double* current);
+/* Slice count, particle count and precision of a beam. */
 g4h_status g4h_beam_info(const g4h_beam* beam, int* nslice, int64_t* total_particles,
                          int* precision);
 
