@@ -56,7 +56,7 @@ def test_adi_variants_bitexact(prec, ngrid):
 
 
 @pytest.mark.parametrize("variant,ngrid", [(0, 33), (1, 33), (1, 64), (1, 65), (1, 400),
-                                            (2, 33), (2, 65), (2, 400)])
+                                            (2, 33), (2, 65), (2, 400), (1, 1024), (2, 1024)])
 def test_adi_variant_oracle_fp64(variant, ngrid):
     """Deposition + ADI step with the source on, each variant pinned."""
     print(f"case adi_variant_oracle variant={variant} ngrid={ngrid}", flush=True)
