@@ -490,10 +490,8 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     const int col = c0 + sc_col;
     const bool scol = sp && col >= rc.x && col < rc.z;
     const double dx = col + shift;
-    fftr::static_for<0, G / RPI>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      const int row = i * RPI + sc_row;
-      const int64_t gi = static_cast<int64_t>(row) * G + col;
+    // the stored value of tile cell (row, this thread's column)
+    auto out = [&](int row, int64_t gi) {
       T2 o = buf[row * TP + sc_col];
       o.x *= sc;
       o.y *= sc;
@@ -506,24 +504,40 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
         o.x += static_cast<P>(wv.x);
         o.y += static_cast<P>(wv.y);
       }
+      return o;
+    };
+    // The thread's column is fixed, so its x moments are dx and dx^2 times
+    // its sum of |u|^2, taken once after the loop.
+    double sw = 0.0;
+    fftr::static_for<0, G / RPI>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int row = i * RPI + sc_row;
+      const int64_t gi = static_cast<int64_t>(row) * G + col;
+      const T2 o = out(row, gi);
       st_stream(&plane[gi], o);
       if (partials) {
         const double re = o.x, im = o.y;
         const double wei = re * re + im * im;
         const double dy = row + shift;
-        acc[0] += wei;
-        acc[1] += dx * wei;
-        acc[2] += dx * dx * wei;
+        sw += wei;
         acc[3] += dy * wei;
         acc[4] += dy * dy * wei;
         acc[5] += re;
         acc[6] += im;
-        if (gi == icenter) {
-          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
-          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
-        }
       }
     });
+    if (partials) {
+      acc[0] = sw;
+      acc[1] = dx * sw;
+      acc[2] = dx * dx * sw;
+      // the centre cell, recomputed as stored by the thread that holds it
+      const int crow = static_cast<int>(icenter / G), ccol = static_cast<int>(icenter % G);
+      if (col == ccol && crow >= sc_row && (crow - sc_row) % RPI == 0) {
+        const T2 o = out(crow, icenter);
+        partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = o.x;
+        partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = o.y;
+      }
+    }
   } else if (valid) {
     const double dx = c + shift;
     fftr::static_for<0, R>([&](auto ic) {
