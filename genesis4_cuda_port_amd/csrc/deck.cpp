@@ -176,6 +176,7 @@ std::map<std::string, Element> parse_lattice(const std::string& text) {
     const std::string label = upper(trim(stmt.substr(0, colon)));
     std::string type = upper(trim(stmt.substr(colon + 1, eq - colon - 1)));
     std::string rest = trim(stmt.substr(eq + 1));
+    if (label.empty()) throw std::runtime_error("lattice: element without a label in '" + stmt + "'");
     if (rest.size() < 2 || rest.front() != '{' || rest.back() != '}')
       throw std::runtime_error("lattice: element '" + label + "' needs a {...} body");
     rest = rest.substr(1, rest.size() - 2);
@@ -204,6 +205,8 @@ std::map<std::string, Element> parse_lattice(const std::string& text) {
           name = trim(item.substr(star + 1));
           if (rep < 0) throw std::runtime_error("lattice: negative repeat in line '" + label + "'");
         }
+        if (name.empty())
+          throw std::runtime_error("lattice: empty element name in line '" + label + "'");
         el.line.emplace_back(rep, upper(name));
       } else {
         const size_t e = item.find('=');

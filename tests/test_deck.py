@@ -105,6 +105,9 @@ def test_lattice_output_step_and_zstop():
     ("A: WIGGLER = {l=1};", "unknown type"),
     ("A: DRIFT = {l=1}; A: DRIFT = {l=2};", "defined twice"),
     ("A: DRIFT = {l=x};", "expects a number"),
+    ("A: DRIFT = {l=1}; : DRIFT = {l=2};", "without a label"),
+    ("A: LINE = {3*};", "empty element name"),
+    ("A: UNDULATOR = {lambdau=0.015, nwig=", "needs a {...} body"),
 ])
 def test_lattice_errors(text, msg):
     with pytest.raises(G4Error) as e:
