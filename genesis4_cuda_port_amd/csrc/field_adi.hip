@@ -678,10 +678,7 @@ void adi_prepare(Field* f, double delz) {
   const double key = delz;
   auto it = f->delz_tables.find(key);
   if (it == f->delz_tables.end()) {
-    if (f->delz_tables.size() >= 64) {  // many distinct step lengths: start over
-      G4_HIP(hipStreamSynchronize(f->ctx->stream));
-      f->delz_tables.clear();
-    }
+    if (f->delz_tables.size() >= Field::kMaxDelzTables) f->evict_delz_tables();
     const double rtmp = 0.25 * delz / (f->xks * f->dgrid * f->dgrid);
     using cd = std::complex<double>;
     std::vector<double> mupp(G), mmid(G), mlow(G);

@@ -806,10 +806,7 @@ void fft_prepare(Field* f, double delz) {
   if (!f->fft_prop || delz != f->fft_delz) {
     auto it = f->delz_tables.find(delz);
     if (it == f->delz_tables.end()) {
-      if (f->delz_tables.size() >= 64) {  // many distinct step lengths: start over
-        G4_HIP(hipStreamSynchronize(f->ctx->stream));
-        f->delz_tables.clear();
-      }
+      if (f->delz_tables.size() >= Field::kMaxDelzTables) f->evict_delz_tables();
       // k2 at frequency index m: dx(m) = ((m - (G+1)/2) mod G) - (G-1)/2 and
       // k2 = -(dx^2 + dy^2) dk^2/(2 xks) (FieldSolverFFTCUDA.cpp:297-309);
       // the x (or y) factor is exp(-i dx^2 dk^2 delz / (2 xks)).

@@ -317,7 +317,21 @@ struct Field {
     DevBuf<double2> dev;
   };
   std::map<double, DelzTable> delz_tables;
+  static constexpr size_t kMaxDelzTables = 64;
   int generation = 0;  // bumped when device tables a captured run may use are released
+  // Many distinct step lengths: release every table and start over.  Graphs
+  // captured earlier hold pointers into these tables, so the generation is
+  // bumped and their replay refused; a capture in progress cannot release
+  // them (its earlier nodes use them) and fails instead.
+  void evict_delz_tables() {
+    require(!ctx->capturing,
+            "a captured run may use at most 64 distinct step lengths per field");
+    G4_HIP(hipStreamSynchronize(ctx->stream));
+    delz_tables.clear();
+    adi_c = adi_cbet = adi_cwet = nullptr;
+    fft_prop = nullptr;
+    ++generation;
+  }
   // ADI
   double adi_delz = 0.0;
   const double2* adi_c = nullptr;  // current tables [ngrid] (views into delz_tables)

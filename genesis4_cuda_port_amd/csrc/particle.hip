@@ -955,10 +955,12 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
     scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
     int4 bb = prm.dep.bbox[s];
-    if (bb.x > bb.z && scl != P(0)) {
+    if (K == 1 && bb.x > bb.z && scl != P(0)) {
       // No footprint from a previous step (first deposition of a run): take
       // it from the current positions, which move by far less than the
-      // window margin within one step.
+      // window margin within one step.  A split slice (K > 1) has it taken
+      // by split_prepass_kernel before any part moves a particle, so that
+      // all parts place the same window.
       if (tid == 0) {
         s_bb[0] = s_bb[1] = 0x7fffffff;
         s_bb[2] = s_bb[3] = -1;
@@ -1267,25 +1269,63 @@ __global__ void slice_kick_kernel(P* g, const int32_t* off, const double* dgamma
   for (int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) g[i] += d;
 }
 
-// Zero the cells the previous deposition left in each slice's source plane
-// (the source rectangle) before a split pass deposits into it.
+// Set-up of a split pass (K workgroups per slice), one workgroup per slice,
+// run before particle_kernel on the same stream: zero the cells the previous
+// deposition left in the slice's source plane (the source rectangle), and on a
+// run's first deposition (no footprint yet) take the footprint of the
+// current positions into dep.bbox.  Doing the latter here rather than in each
+// part keeps every part's window identical: inside particle_kernel the parts
+// of a slice run concurrently and overwrite their positions while a sibling
+// would still be scanning them.
 template <class P, bool TAB>
-__global__ void zero_rect_kernel(ParticleParams prm_arg, const ParticleParams* __restrict__ tab) {
+__global__ void split_prepass_kernel(BeamPtr<P> b, const int32_t* off, ParticleParams prm_arg,
+                                     const ParticleParams* __restrict__ tab) {
   const ParticleParams& prm = slice_params<TAB, P>(prm_arg, tab);
   using T2 = typename V2<P>::T;
   const int s = blockIdx.x;
+  const int tid = threadIdx.x;
   const int ng = prm.dep.ngrid;
   T2* src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
   const int4 old = prm.dep.rect[s];
   const int ow = old.z - old.x, oh = old.w - old.y;
-  if (ow <= 0 || oh <= 0) return;
-  for (int k = threadIdx.x; k < ow * oh; k += blockDim.x) {
-    const int cy = old.y + k / ow, cx = old.x + k % ow;
-    T2 z;
-    z.x = 0;
-    z.y = 0;
-    src[cx + static_cast<int64_t>(cy) * ng] = z;
+  if (ow > 0 && oh > 0) {
+    for (int k = tid; k < ow * oh; k += blockDim.x) {
+      const int cy = old.y + k / ow, cx = old.x + k % ow;
+      T2 z;
+      z.x = 0;
+      z.y = 0;
+      src[cx + static_cast<int64_t>(cy) * ng] = z;
+    }
   }
+  const int4 bb = prm.dep.bbox[s];
+  const P scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
+  if (!(bb.x > bb.z && scl != P(0))) return;
+  __shared__ int s_bb[4];
+  if (tid == 0) {
+    s_bb[0] = s_bb[1] = 0x7fffffff;
+    s_bb[2] = s_bb[3] = -1;
+  }
+  __syncthreads();
+  int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+  for (int64_t k = off[s] + tid; k < off[s + 1]; k += blockDim.x) {
+    int ix, iy;
+    P wx, wy;
+    if (cell_of<P>(b.x[k], b.y[k], P(prm.dep.gridmax), P(prm.dep.dgrid), P(prm.dep.inv_dgrid), ng,
+                   ix, iy, wx, wy)) {
+      mnx = min(mnx, ix);
+      mny = min(mny, iy);
+      mxx = max(mxx, ix + 1);
+      mxy = max(mxy, iy + 1);
+    }
+  }
+  atomicMin(&s_bb[0], mnx);
+  atomicMin(&s_bb[1], mny);
+  atomicMax(&s_bb[2], mxx);
+  atomicMax(&s_bb[3], mxy);
+  __syncthreads();
+  // stays empty when no particle is on the grid: every part then deposits
+  // through the global fallback
+  if (tid == 0 && s_bb[0] <= s_bb[2]) prm.dep.bbox[s] = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
 }
 
 // Workgroups per slice: a pass over few slices is split so that about two
@@ -1328,8 +1368,8 @@ void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
                       b->split_cnt.get()};
     if (DEP) {
       ctx->launch(G4H_K_OTHER, [&] {
-        hipLaunchKernelGGL((zero_rect_kernel<P, TAB>), dim3(b->nslice), dim3(256), 0,
-                           ctx->stream, p, tab);
+        hipLaunchKernelGGL((split_prepass_kernel<P, TAB>), dim3(b->nslice), dim3(256), 0,
+                           ctx->stream, bp, b->off.get(), p, tab);
       });
     }
   }

@@ -19,6 +19,7 @@
 // eager pass, g4h_run_capture records the initial record and every step of
 // the run once; g4h_run_replay then enqueues the whole run as one graph
 // launch on state the caller has re-initialised.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -179,6 +180,21 @@ g4h_status g4h_run_capture(g4h_run* r) {
           "advance the run to its end once before capturing it");
   Context* ctx = r->ctx;
   ctx->activate();
+  // Every solver table the graph will point at is built before the capture:
+  // a capture cannot allocate or upload tables, nor release those its earlier
+  // nodes use.
+  {
+    std::vector<double> lens;
+    for (const g4h_step& st : r->steps) lens.push_back(st.delz);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    require(lens.size() <= Field::kMaxDelzTables,
+            "a captured run may use at most 64 distinct step lengths per field");
+    size_t missing = 0;
+    for (double d : lens) missing += r->f->delz_tables.count(d) ? 0 : 1;
+    if (r->f->delz_tables.size() + missing > Field::kMaxDelzTables) r->f->evict_delz_tables();
+    for (double d : lens) field_prepare_solver(r->f, d);
+  }
   G4_HIP(hipStreamSynchronize(ctx->stream));
   ctx->tune_settle();
   if (r->b->prec == G4H_FP64) {  // one parameter block per particle launch

@@ -28,14 +28,17 @@
  *                          conservation; Gaussian-beam diffraction
  *   orc_fft_advance, orc_dft  numpy FFT; 2S identity; odd grids equal the
  *                          textbook Fresnel propagator and numpy-layout filter;
+ *                          even grids equal the textbook step plus a
+ *                          shift-theorem translation of pi dz/(G dgrid^2 ks)
+ *                          cells and a constant phase (filter at the same
+ *                          half-bin frequencies), and a Gaussian mode's
+ *                          centroid drifts by that much per step;
  *                          Gaussian-beam diffraction; ADI ~ FFT on a smooth beam
  *   orc_apply_slippage     integer ring model
  *   orc_diag_*, orc_finalize_*  quiet-start moments and zero bunching;
  *                          shot-noise statistics; Gaussian-mode power, size
  *   orc_run                beam-to-field energy balance (first order in delz)
- * Parity unpinned: the even-grid half-bin offset of the FFT k2 / sigmoid
- * tables follows FieldSolverFFTCUDA.cpp:297-316 and nothing independent fixes
- * it; the physical constants (vacimp, eev, ce) and fc(h) follow upstream
+ * Parity unpinned: the physical constants (vacimp, eev, ce) and fc(h) follow upstream
  * definitions that are absent from the reference.
  */
 #ifndef G4ORACLE_H_

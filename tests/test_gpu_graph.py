@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import cases
-from cases import LAMBDA0
+from cases import LAMBDA0, quad_step
 from genesis4_cuda_port_amd import api
 from genesis4_cuda_port_amd._abi import G4Error
 
@@ -69,6 +69,45 @@ def test_graph_replay_matches_eager(prec, kind):
             # eager work on the same stream between replays
             b.diag_raw(2)
             ctx.synchronize()
+        run.close()
+        f.close()
+        b.close()
+
+
+@pytest.mark.parametrize("kind", [api.ADI, api.FFT])
+def test_replay_refused_after_table_eviction(kind):
+    """A field keeps solver tables for at most 64 distinct step lengths.  When
+    another run on the same field steps through 65 of them the tables are
+    released and rebuilt, so a graph captured earlier (whose nodes point at the
+    released tables) must refuse to replay; capturing a run with 65 distinct
+    step lengths fails with a status and leaves the stream usable."""
+    print(f"case replay_after_eviction kind={kind}", flush=True)
+    g = 32 if kind == api.FFT else 31
+    c = cases.make_case(nslice=4, npart=256, ngrid=g, nsteps=6, seed_power=1e4)
+    many = api.steps_array([quad_step(delz=cases.LAMBDAU * (1.0 + 1e-3 * k)) for k in range(65)])
+    with api.Context(0) as ctx:
+        b = gpu_beam(ctx, c, api.FP64)
+        f = gpu_field(ctx, c, api.FP64, kind=kind)
+        run = api.Run(ctx, b, f, c.steps, LAMBDA0, c.sample, c.timerun, False, 1)
+        run.advance(len(c.steps))
+        run.capture()
+        f.upload(c.seed_re, c.seed_im, 0)
+        run.replay()  # tables still live: allowed
+        other = api.Run(ctx, b, f, many, LAMBDA0, c.sample, c.timerun, False, 1)
+        assert other.advance(65) == 65
+        f.upload(c.seed_re, c.seed_im, 0)
+        with pytest.raises(G4Error):
+            run.replay()
+        other.close()
+        f.upload(c.seed_re, c.seed_im, 0)
+        third = api.Run(ctx, b, f, many, LAMBDA0, c.sample, c.timerun, False, 1)
+        assert third.advance(65) == 65
+        with pytest.raises(G4Error):
+            third.capture()
+        third.close()
+        # the stream is out of capture mode: eager work still runs
+        b.diag_raw(1)
+        ctx.synchronize()
         run.close()
         f.close()
         b.close()

@@ -472,6 +472,114 @@ def test_fft_propagator_and_filter_match_textbook_layout_on_odd_grids(G):
     assert np.linalg.norm(want - 2 * s) > 0.1 * np.linalg.norm(2 * s)
 
 
+def _even_grid_step(u, G, dgrid, ks, dz):
+    """The closed form of one free-space step of the reference's even-G table
+    layout (FieldSolverFFTCUDA.cpp:297-309).  With f = G fftfreq(G) the signed
+    integer frequency of FFT bin m (Nyquist at -G/2), the table gives bin m the
+    offset dx = f + 1/2, so the propagator exp(-i dx^2 dk^2 dz/(2 ks)) factors
+    into three parts, per axis:
+      exp(-i f^2 dk^2 dz/(2 ks))   the textbook Fresnel propagator,
+      exp(-i f dk^2 dz/(2 ks))     = exp(-2 pi i f s/G): by the shift theorem a
+                                   translation by s = pi dz/(G dgrid^2 ks) cells
+                                   towards larger cell index,
+      exp(-i dk^2 dz/(8 ks))       a constant phase.
+    Returns (the stepped field, s)."""
+    k = 2 * np.pi * np.fft.fftfreq(G, d=dgrid)
+    fresnel = np.exp(-1j * (k[None, :] ** 2 + k[:, None] ** 2) * dz / (2 * ks))
+    s = np.pi * dz / (G * dgrid ** 2 * ks)
+    f = G * np.fft.fftfreq(G)
+    shift = np.exp(-2j * np.pi * (f[None, :] + f[:, None]) * s / G)
+    dk = 2 * np.pi / (G * dgrid)
+    const = np.exp(-1j * 2 * dk ** 2 * dz / (8 * ks))
+    return np.fft.ifft2(np.fft.fft2(u) * fresnel * shift) * const, s
+
+
+def _table_layout_step(u, G, dgrid, ks, dz, centre, half):
+    """One step with a k^2 table built from the index map
+    dx = ((m - centre) mod G) - half (the reference uses centre = (G+1)/2 in
+    integer arithmetic and half = (G-1)/2 in real arithmetic); used below to
+    show that plausible misreadings of the layout fail the closed form."""
+    dk = 2 * np.pi / (G * dgrid)
+    m = np.arange(G)
+    dx = ((m - centre) % G) - half
+    ph = -(dx[None, :] ** 2 + dx[:, None] ** 2) * dk * dk / (2 * ks) * dz
+    return np.fft.ifft2(np.fft.fft2(u) * np.exp(1j * ph))
+
+
+@pytest.mark.parametrize("G", [16, 256])
+def test_fft_propagator_even_grid_is_textbook_step_plus_translation(G):
+    """Even grids (the benchmark's G = 256): the oracle's FFT step must equal
+    the textbook Fresnel step followed by a translation of s cells along x and
+    y and a constant phase (see _even_grid_step).  The filter must be the
+    textbook sigmoid evaluated at the same half-bin-offset frequencies.  The
+    reference's layout is (G+1)/2 integer, (G-1)/2 real; reading either as the
+    other kind, or centring at G/2 - 1, must fail the closed form."""
+    gm = 2e-4 if G == 16 else 0.4e-3
+    dgrid = 2 * gm / (G - 1)
+    ks = 2 * np.pi / LAMBDA0
+    rng = np.random.default_rng(G + 1)
+    u = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+    dz = 0.7 if G == 16 else 0.026 * 50
+    want, s = _even_grid_step(u, G, dgrid, ks, dz)
+    assert s > 1e-3  # the translation is visible at this step length
+    a = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, u.real.ravel(), u.imag.ravel())
+    L.orc_fft_advance(a.ptr, None, None, dz, 0, 1.0, 1.0, 1.0)
+    got = (a.re + 1j * a.im).reshape(G, G)
+    scale = np.abs(u).max()
+    assert np.abs(got - want).max() < 1e-11 * scale
+    # plausible misreadings of the layout differ from the closed form
+    for centre, half in [(G // 2 - 1, (G - 1) / 2), ((G + 1) // 2, (G - 1) // 2),
+                         ((G + 1) // 2, G / 2), ((G + 1) // 2, (G - 2) / 2)]:
+        alt = _table_layout_step(u, G, dgrid, ks, dz, centre, half)
+        assert np.abs(alt - want).max() > 1e-6 * scale, (centre, half)
+    # the reference's own reading reproduces it (the closed form is not loose)
+    ref = _table_layout_step(u, G, dgrid, ks, dz, (G + 1) // 2, (G - 1) / 2)
+    assert np.abs(ref - want).max() < 1e-11 * scale
+    # filter: textbook sigmoid at frequencies (f + 1/2)/G per axis
+    xc, yc, sig = 0.2, 0.35, 0.05
+    f = G * np.fft.fftfreq(G)
+    fx = (f + 0.5) / G / xc
+    fy = (f + 0.5) / G / yc
+    r = np.sqrt(fx[None, :] ** 2 + fy[:, None] ** 2)
+    filt = 1.0 / (1.0 + np.exp((r - 1.0) / sig))
+    src = rng.standard_normal((G, G)) + 1j * rng.standard_normal((G, G))
+    z = np.zeros(G * G)
+    b = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, z, z)
+    L.orc_fft_advance(b.ptr, dp(np.ascontiguousarray(src.real.ravel())),
+                      dp(np.ascontiguousarray(src.imag.ravel())), dz, 1, xc, yc, sig)
+    wantf = 2 * np.fft.ifft2(filt * np.fft.fft2(src))
+    assert np.abs((b.re + 1j * b.im).reshape(G, G) - wantf).max() < 1e-11 * np.abs(src).max()
+
+
+def test_fft_even_grid_gaussian_centroid_drifts():
+    """The translation seen on a physical beam: at the benchmark grid (G = 256,
+    gridmax 0.4 mm) a centred Gaussian mode's intensity centroid moves by
+    s dgrid per step along x and y, s = pi dz/(G dgrid^2 ks) (about 0.002
+    cells per 26 mm step: 2 cells over a 1000-step lattice), while an odd grid
+    keeps it centred."""
+    for G, moves in [(256, True), (255, False)]:
+        gm = 0.4e-3
+        dgrid = 2 * gm / (G - 1)
+        ks = 2 * np.pi / LAMBDA0
+        re, im = api.seed_gaussian(1, G, gm, LAMBDA0, 1.0, 60e-6)
+        of = OField(1, G, gm, LAMBDA0, LAMBDA0, 1, re, im)
+        dz, nstep = LAMBDAU * 20, 10
+        for _ in range(nstep):
+            L.orc_fft_advance(of.ptr, None, None, dz, 0, 1.0, 1.0, 1.0)
+        p = (of.re ** 2 + of.im ** 2).reshape(G, G)
+        xs = np.linspace(-gm, gm, G)
+        cx = np.sum(p.sum(0) * xs) / p.sum()
+        cy = np.sum(p.sum(1) * xs) / p.sum()
+        s = np.pi * dz / (G * dgrid ** 2 * ks)
+        want = nstep * s * dgrid if moves else 0.0
+        assert cx == pytest.approx(want, abs=1e-6 * dgrid)
+        assert cy == pytest.approx(want, abs=1e-6 * dgrid)
+        if moves:
+            assert want > 0.1 * dgrid
+            # the drift per 26 mm step of the benchmark lattice
+            assert np.pi * LAMBDAU / (G * dgrid ** 2 * ks) == pytest.approx(0.002025, rel=1e-3)
+
+
 def _gaussian_rms(of, G, gm):
     p = (of.re ** 2 + of.im ** 2).reshape(G, G)
     xs = np.linspace(-gm, gm, G)
