@@ -163,6 +163,7 @@ PROTOTYPES = {
                                      C.POINTER(C.c_int)]),
     "g4h_beam_cell_index": (C.c_int, [_vp, _vp, _vp, _ip]),
     "g4h_diag_beam_raw": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "g4h_diag_beam_raw_noalloc": (C.c_int, [_vp, _vp, C.c_int, _dp]),
     "g4h_diag_field_raw": (C.c_int, [_vp, _vp, _dp]),
     "g4h_diag_finalize_beam": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp]),
     "g4h_diag_finalize_field": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _dp,

@@ -193,6 +193,12 @@ class Beam:
         check(lib().g4h_diag_beam_raw(self.ctx.h, self.h, nharm, _dp(out)))
         return out.reshape(self.nslice, beam_raw_len(nharm))
 
+    def diag_raw_noalloc(self, nharm: int = 1) -> np.ndarray:
+        """diag_raw through the entry point that allocates no device memory."""
+        out = np.empty(self.nslice * beam_raw_len(nharm))
+        check(lib().g4h_diag_beam_raw_noalloc(self.ctx.h, self.h, nharm, _dp(out)))
+        return out.reshape(self.nslice, beam_raw_len(nharm))
+
 
 class Field:
     def __init__(self, ctx: Context, precision: int, nslice: int, ngrid: int, gridmax: float,

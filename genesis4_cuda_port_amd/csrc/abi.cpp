@@ -371,6 +371,8 @@ g4h_status g4h_context_create(int device, g4h_context** out) {
   try {
     c->activate();
     G4_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // device address of the fixed scratch array of g4h_diag_beam_raw_noalloc
+    scratch_base(device);
   } catch (...) {
     delete c;
     throw;
@@ -882,6 +884,17 @@ g4h_status g4h_diag_beam_raw(g4h_context* ctx, g4h_beam* b, int nharm, double* r
   launch_particle(ctx, b, p);
   G4_HIP(hipMemcpyAsync(raw, d.get(), n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   G4_HIP(hipStreamSynchronize(ctx->stream));
+  G4_API_END
+}
+
+g4h_status g4h_diag_beam_raw_noalloc(g4h_context* ctx, g4h_beam* b, int nharm, double* raw) {
+  G4_API_BEGIN
+  require(ctx && raw, "null argument");
+  check_beam(b);
+  require(nharm >= 1 && nharm <= 8, "nharm must be in [1, 8]");
+  ctx->activate();
+  if (b->nslice == 0) return G4H_OK;
+  diag_beam_raw_noalloc(ctx, b, nharm, raw);
   G4_API_END
 }
 

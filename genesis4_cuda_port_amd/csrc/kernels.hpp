@@ -116,6 +116,13 @@ struct ParticleParams {
 void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
                      const ParticleParams* d_table = nullptr);
 void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell);
+// Beam sums into the library's fixed device array (no allocation): blocking,
+// raw[nslice * G4H_BEAM_RAW(nharm)] on the host; fails with
+// G4H_ERR_INVALID_ARGUMENT when the pass needs more than kNoallocScratchBytes.
+constexpr size_t kNoallocScratchBytes = size_t(8) << 20;
+void diag_beam_raw_noalloc(Context* ctx, Beam* b, int nharm, double* raw);
+// Device address of that array on `device` (current), looked up once.
+void* scratch_base(int device);
 // gamma += d_dgamma[slice] (device, nslice values) for every particle
 void launch_slice_kick(Context* ctx, Beam* b, const double* d_dgamma);
 

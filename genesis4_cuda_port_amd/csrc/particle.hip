@@ -38,6 +38,10 @@
 // would cancel 240 - 240 in FP32.
 #include <algorithm>
 #include <cmath>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -1409,6 +1413,33 @@ void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
   return dispatch_nh<P, 0, false, false>(ctx, b, p, bp, nullptr);
 }
 
+__global__ void fill_i32_kernel(int32_t* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Beam-sums pass (ST_DIAG only) whose scratch the caller provides: d_prm is
+// the FP64 parameter block in device memory (unused in MIXED, which takes
+// it by value), sp the split pass's partials and counters for K > 1.
+template <class P, int NH>
+void launch_diag_given(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
+                       const ParticleParams* d_prm, SplitScratch sp, int K) {
+  ctx->launch(G4H_K_PARTICLE, [&] {
+    hipLaunchKernelGGL((particle_kernel<P, 1, false, NH, false>), dim3(b->nslice, K),
+                       dim3(kThreads), 0, ctx->stream, bp, b->off.get(), p, d_prm, sp);
+  });
+}
+
+template <class P>
+void diag_given(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
+                const ParticleParams* d_prm, SplitScratch sp, int K) {
+  const int nh = p.diag.nharm;
+  if (nh == 1) return launch_diag_given<P, 1>(ctx, b, p, bp, d_prm, sp, K);
+  if (nh == 2) return launch_diag_given<P, 2>(ctx, b, p, bp, d_prm, sp, K);
+  if (nh <= 4) return launch_diag_given<P, 4>(ctx, b, p, bp, d_prm, sp, K);
+  return launch_diag_given<P, 8>(ctx, b, p, bp, d_prm, sp, K);
+}
+
 }  // namespace
 
 void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell) {
@@ -1457,6 +1488,108 @@ void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
                       b->theta.get()};
     dispatch<float>(ctx, b, p, bp, d_table);
   }
+}
+
+// ---- beam sums without allocation ----------------------------------------
+// Device memory for callers that cannot allocate (code under graph capture,
+// tight serving loops): the FP64 parameter block, the per-slice sums before
+// their copy to the host, and the split pass's per-part partials and
+// per-slice arrival counters all live in this array.  Each GPU the library
+// runs on holds its own copy until the process ends.  Host code must use the
+// address from hipGetSymbolAddress (scratch_base), never the array's name.
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;          // per device, from hipGetSymbolAddress
+std::vector<std::unique_ptr<std::mutex>> g_scratch_use;  // one caller at a time per device
+
+size_t align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+}  // namespace
+
+void* scratch_base(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev.size() <= static_cast<size_t>(device)) {
+    g_scratch_dev.resize(static_cast<size_t>(device) + 1, nullptr);
+    g_scratch_use.resize(static_cast<size_t>(device) + 1);
+  }
+  if (!g_scratch_dev[static_cast<size_t>(device)]) {
+    // the caller has made `device` current
+    void* ptr = nullptr;
+    G4_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[static_cast<size_t>(device)] = ptr;
+    g_scratch_use[static_cast<size_t>(device)] = std::make_unique<std::mutex>();
+  }
+  return g_scratch_dev[static_cast<size_t>(device)];
+}
+
+size_t diag_beam_noalloc_bytes(Context* ctx, const Beam* b, int nharm, int* split) {
+  const int K = split_of(ctx, b);
+  if (split) *split = K;
+  size_t n = align256(sizeof(ParticleParams));
+  n += align256(static_cast<size_t>(b->nslice) * G4H_BEAM_RAW(nharm) * sizeof(double));
+  if (K > 1) {
+    const size_t parts = static_cast<size_t>(b->nslice) * K;
+    n += align256(parts * kRedStride * sizeof(double));
+    n += align256(parts * sizeof(int4));
+    n += align256(static_cast<size_t>(b->nslice) * sizeof(int32_t));
+  }
+  return n;
+}
+
+void diag_beam_raw_noalloc(Context* ctx, Beam* b, int nharm, double* raw) {
+  int K = 1;
+  const size_t need = diag_beam_noalloc_bytes(ctx, b, nharm, &K);
+  if (need > kNoallocScratchBytes)
+    fail(G4H_ERR_INVALID_ARGUMENT,
+         "g4h_diag_beam_raw_noalloc: " + std::to_string(b->nslice) + " slices with " +
+             std::to_string(nharm) + " harmonics need " + std::to_string(need) +
+             " bytes of scratch, more than its fixed " + std::to_string(kNoallocScratchBytes) +
+             " (use g4h_diag_beam_raw)");
+  char* base = static_cast<char*>(scratch_base(ctx->device));
+  std::mutex* use;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    use = g_scratch_use[static_cast<size_t>(ctx->device)].get();
+  }
+  std::lock_guard<std::mutex> hold(*use);
+  size_t at = 0;
+  auto* d_prm = reinterpret_cast<ParticleParams*>(base + at);
+  at += align256(sizeof(ParticleParams));
+  auto* d_raw = reinterpret_cast<double*>(base + at);
+  const size_t nraw = static_cast<size_t>(b->nslice) * G4H_BEAM_RAW(nharm);
+  at += align256(nraw * sizeof(double));
+  SplitScratch sp{nullptr, nullptr, nullptr, nullptr};
+  if (K > 1) {
+    const size_t parts = static_cast<size_t>(b->nslice) * K;
+    sp.red = reinterpret_cast<double*>(base + at);
+    at += align256(parts * kRedStride * sizeof(double));
+    sp.fb = reinterpret_cast<int4*>(base + at);
+    at += align256(parts * sizeof(int4));
+    sp.cnt = reinterpret_cast<int32_t*>(base + at);
+    // the arrival counters start at zero (a fill kernel, not a memset node)
+    const int n = b->nslice;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, sp.cnt,
+                       n, 0);
+    G4_HIP(hipGetLastError());
+  }
+  ParticleParams p{};
+  p.gamma0 = b->gamma_ref;
+  p.stages = ST_DIAG;
+  p.diag.raw = d_raw;
+  p.diag.nharm = nharm;
+  if (b->prec == G4H_FP64) {
+    G4_HIP(hipMemcpyAsync(d_prm, &p, sizeof(p), hipMemcpyHostToDevice, ctx->stream));
+    BeamPtr<double> bp{b->xd.get(), b->yd.get(), b->pxd.get(), b->pyd.get(), b->gd.get(),
+                       b->theta.get()};
+    diag_given<double>(ctx, b, p, bp, d_prm, sp, K);
+  } else {
+    BeamPtr<float> bp{b->xf.get(), b->yf.get(), b->pxf.get(), b->pyf.get(), b->gf.get(),
+                      b->theta.get()};
+    diag_given<float>(ctx, b, p, bp, nullptr, sp, K);
+  }
+  G4_HIP(hipMemcpyAsync(raw, d_raw, nraw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  G4_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace g4

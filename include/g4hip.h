@@ -217,6 +217,22 @@ g4h_status g4h_beam_cell_index(g4h_context* ctx, g4h_beam* beam, const g4h_field
 g4h_status g4h_diag_beam_raw(g4h_context* ctx, g4h_beam* beam, int nharm, double* beam_raw);
 g4h_status g4h_diag_field_raw(g4h_context* ctx, g4h_field* field, double* field_raw);
 
+/* g4h_diag_beam_raw without any device allocation, for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same results as
+ * g4h_diag_beam_raw, bit for bit.  Its device memory (the FP64 parameter
+ * block, the per-slice sums before the copy to the host, the split pass's
+ * per-part partials and arrival counters) is one fixed 8 MiB array per GPU,
+ * held until the process ends: nslice * (23 + 2*nharm) doubles plus, for
+ * beams of few slices, ~60 doubles per workgroup of the split pass must fit,
+ * about 40 000 slices at nharm = 1; larger beams get G4H_ERR_INVALID_ARGUMENT
+ * before any work is enqueued.  Every call on a GPU shares that GPU's array,
+ * which keeps the last call's contents; the library serialises the calls
+ * made through it (blocking, one at a time per GPU), and work of any other
+ * process or library on the same memory would race. */
+g4h_status g4h_diag_beam_raw_noalloc(g4h_context* ctx, g4h_beam* beam, int nharm,
+                                     double* beam_raw);
+
 /* Host finalisation (DiagnosticCUDA.cpp:509-628 and 685-790): raw sums to
  * the named per-slice quantities (layouts in g4hip_types.h) and globals. */
 g4h_status g4h_diag_finalize_beam(int nslice, int nharm, const double* beam_raw,
