@@ -145,10 +145,11 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int,
     and the GPU runs' records of those slices compare with these."""
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle
+    # every core this process may use: the CPU share the host gives the job
+    # (OMP_NUM_THREADS where the host sets it, else the affinity mask)
+    avail = len(os.sched_getaffinity(0))
     if threads <= 0:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    if threads <= 0:
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
     _oracle.lib().orc_set_threads(threads)
     sub = beam.copy()
     npart = int(beam.offsets[1] - beam.offsets[0])
@@ -167,7 +168,8 @@ def cpu_baseline(beam, steps, ngrid, gridmax, nslice_cpu: int, nsteps_cpu: int,
     _, braw, fraw = _oracle.run(sub, of, timed, gamma0=GAMMA0, lambda0=LAMBDA0, solver=1)
     dt = time.perf_counter() - t0
     res = {"value": n * nsteps_cpu / dt, "unit": "particle-slice-steps/s", "cores": threads,
-           "kind": "port",
+           "kind": "port", "host_nproc": os.cpu_count(), "affinity_cpus": avail,
+           "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
            "sample": f"FP64 oracle, first {nslice_cpu} of 4096 slices x {npart} particles, "
                      f"ngrid {ngrid} FFT, {nsteps_cpu} steps after 1 untimed ({dt:.1f} s)"}
     return (res, braw[-1], fraw[-1]) if keep else res
@@ -185,6 +187,86 @@ def slice_quantities(braw, fraw, current, ngrid, gridmax):
 def rel_errors(got, ref):
     return {f"{k}_rel": float(np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300))
             for k in ref}
+
+
+def _run_records(prec, beam, steps, ngrid, gridmax, re, im, solver, timerun):
+    """Records of a whole run on the GPU (every step an output step)."""
+    from genesis4_cuda_port_amd import api
+    S = beam.offsets.size - 1
+    with api.Context(0) as ctx:
+        b = api.Beam(ctx, prec, beam.offsets, beam.current, GAMMA0)
+        b.upload(beam.x, beam.y, beam.px, beam.py, beam.gamma, beam.theta)
+        f = api.Field(ctx, prec, S, ngrid, gridmax, LAMBDA0, LAMBDA0)
+        f.upload(re, im, 0)
+        f.init_solver(solver)
+        run = api.Run(ctx, b, f, steps, LAMBDA0, 1.0, timerun, False, 1)
+        run.advance(len(steps))
+        braw, fraw = run.download_diag()
+        run.close()
+        f.close()
+        b.close()
+    return braw, fraw
+
+
+def precision_along_z(nslice_td: int = 512):
+    """MIXED against FP64 over the whole 10 x 100-period lattice: per record,
+    the normwise relative difference of slice power, bunching and energy
+    spread, for (a) the steady-state run of 1 slice x 8192 particles, ngrid
+    151, ADI, 1 kW seed, and (b) the time-dependent SASE workload of the
+    benchmark (Gaussian current, chirp, shot noise, ngrid 256, FFT) on a
+    window of nslice_td slices at the bunch centre.  Reports the errors every
+    50 steps and the largest z up to which all three stay within 1e-3
+    (records after the first step; bunching relative to max(|b|, 1/sqrt(Ne)),
+    the shot-noise level)."""
+    from genesis4_cuda_port_amd import api
+    steps = api.lattice_fodo(LAMBDAU, [AW] * 10, 100, 46, 3, 1.5, 1)
+    dz = LAMBDAU
+    out = {}
+    cases = []
+    b0 = api.load_beam(nslice=1, npart=8192, nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+                       emitn=0.4e-6, beta=15.0, slicelength=LAMBDA0, current=3000.0,
+                       shotnoise=False, seed=1234567)
+    re0, im0 = api.seed_gaussian(1, 151, 0.3e-3, LAMBDA0, 1e3, 50e-6)
+    cases.append(("steady_state_1x8192_adi151", b0, 151, 0.3e-3, re0, im0, api.ADI, False))
+    S = nslice_td
+    s = np.arange(S, dtype=np.float64) + 2048 - S / 2
+    cur = 4000.0 * np.exp(-0.5 * ((s - 2047.5) / 1000.0) ** 2)
+    b2 = api.load_beam(nslice=S, npart=16384, nbins=4, gamma0=GAMMA0, espread_rel=1e-4,
+                       emitn=0.4e-6, beta=15.0, slicelength=LAMBDA0, current=cur, shotnoise=True,
+                       seed=1234569, chirp_rel=1e-4 * S / 4096)
+    z2 = np.zeros(S * 256 * 256)
+    cases.append((f"sase_{S}x16384_fft256", b2, 256, 0.4e-3, z2, z2, api.FFT, True))
+    for name, beam, G, gm, re, im, solver, td in cases:
+        q = {}
+        for pname, prec in (("fp64", api.FP64), ("mixed", api.MIXED)):
+            braw, fraw = _run_records(prec, beam, steps, G, gm, re, im, solver, td)
+            rows = [slice_quantities(braw[k], fraw[k], beam.current, G, gm)
+                    for k in range(braw.shape[0])]
+            q[pname] = rows
+        # Bunching below the shot-noise level 1/sqrt(Ne) of a slice (Ne
+        # electrons) carries no physical information (a mirrored quiet start
+        # has a bunching of round-off size until the field has acted), so
+        # its error is taken relative to max(|b|, 1/sqrt(Ne)).
+        ne = float(np.mean(beam.current)) * LAMBDA0 / 1.602176634e-19 / 2.99792458e8
+        floor = {"power": 1e-300, "bunching": np.sqrt(beam.current.size / ne),
+                 "energyspread": 1e-300}
+        errs = {k: [] for k in ("power", "bunching", "energyspread")}
+        for a, b in zip(q["mixed"], q["fp64"]):
+            for k in errs:
+                den = max(np.linalg.norm(b[k]), floor[k])
+                errs[k].append(float(np.linalg.norm(a[k] - b[k]) / den))
+        n = len(errs["power"])
+        worst = [max(errs[k][r] for k in errs) for r in range(n)]
+        bad = [r for r in range(1, n) if worst[r] > 1e-3]
+        z_ok = (bad[0] - 1) * dz if bad else (n - 1) * dz
+        upto = worst[1:bad[0]] if bad else worst[1:]
+        p64 = [float(np.sum(r["power"])) for r in q["fp64"]]
+        out[name] = {"z_m": [round(r * dz, 4) for r in range(0, n, 50)],
+                     "power_W_fp64": [p64[r] for r in range(0, n, 50)],
+                     **{f"{k}_rel": [errs[k][r] for r in range(0, n, 50)] for k in errs},
+                     "z_within_1e-3_m": z_ok, "lattice_m": (n - 1) * dz,
+                     "max_rel_up_to_z": max(upto) if upto else None}
+    return out
 
 
 def time_dependent_adi(nsteps: int, warmup: int):
@@ -316,6 +398,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adi", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the MIXED-vs-FP64 comparison over the whole lattice")
     ap.add_argument("--batch-runs", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1536)
@@ -431,6 +515,9 @@ def main() -> int:
         one = cpu_baseline(beam, steps, a.ngrid, gridmax, max(1, a.cpu_slices // 32),
                            a.cpu_steps, threads=1)
         out["cpu_baseline"]["single_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
+    if not a.no_precision:
+        log("MIXED vs FP64 along the whole lattice")
+        out["mixed_vs_fp64_along_z"] = precision_along_z()
     if not a.no_adi:
         log("time-dependent ADI run (FP64)")
         out["time_dependent_adi_fp64"] = time_dependent_adi(a.steps, a.warmup)

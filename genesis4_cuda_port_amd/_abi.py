@@ -23,7 +23,7 @@ STATUS_NAMES = {
 }
 FP64, MIXED = 0, 1
 ADI, FFT = 0, 1
-TUNE_ADI, TUNE_SPLIT = 0, 1  # kernel families of g4h_context_set_variant
+TUNE_ADI, TUNE_SPLIT, TUNE_BATCH = 0, 1, 2  # kernel families of g4h_context_set_variant
 
 BEAM_STATS = 23
 FIELD_RAW = 9

@@ -147,30 +147,49 @@ void lmul(M4& m, const M4& e) {
       for (int k = 0; k < 4; ++k) t.v[i][j] += e.v[i][k] * m.v[k][j];
   m = t;
 }
+// Drift of length len in both planes.
+M4 drift(double len) {
+  M4 d = eye4();
+  d.v[0][1] = d.v[2][3] = len;
+  return d;
+}
+// Sector dipole of bend angle phi and radius rho: a rotation on the arc in
+// the bending plane (x), a drift of the arc length in the other (y).  The
+// chicane's dipoles bend alternately by +phi and -phi, each in its own frame,
+// which gives this same matrix for all four.
+M4 sector_bend(double phi, double rho) {
+  M4 b = eye4();
+  const double c = std::cos(phi), s = std::sin(phi);
+  b.v[0][0] = b.v[1][1] = c;
+  b.v[0][1] = rho * s;
+  b.v[1][0] = -s / rho;
+  b.v[2][3] = rho * phi;
+  return b;
+}
+// Thin edge kick of a rectangular dipole face: strength k in x, -k in y.
+M4 edge_kick(double k) {
+  M4 e = eye4();
+  e.v[1][0] = k;
+  e.v[3][2] = -k;
+  return e;
+}
+
+// Four-dipole chicane of total length lt, dipole length lb, outer drifts ld
+// (the reference's TrackBeamCUDA.cpp:93-148): dipole, edge, drift, edge,
+// dipole, centre drift, and the mirror image, followed by a drift of -lt
+// because the transverse tracking still drifts over the chicane's length.
+// The product is taken in that order so that it rounds as the reference's.
 M4 chicane(double angle, double lb, double ld, double lt) {
-  M4 m = eye4(), d1 = eye4(), d2 = eye4(), d3 = eye4(), bp = eye4(), bn = eye4(), ep = eye4(),
-     en = eye4();
-  d1.v[0][1] = d1.v[2][3] = ld / std::cos(angle);
-  d2.v[0][1] = d2.v[2][3] = lt - 4 * lb - 2 * ld;
-  d3.v[0][1] = d3.v[2][3] = -lt;
-  const double radius = lb / std::sin(angle);
-  const double path = radius * angle;
-  bp.v[2][3] = path;
-  bp.v[0][0] = std::cos(angle);
-  bp.v[0][1] = radius * std::sin(angle);
-  bp.v[1][0] = -std::sin(angle) / radius;
-  bp.v[1][1] = std::cos(angle);
-  bn.v[2][3] = path;
-  bn.v[0][0] = std::cos(-angle);
-  bn.v[0][1] = radius * std::sin(-angle) * -1;
-  bn.v[1][0] = -std::sin(-angle) / radius * -1;
-  bn.v[1][1] = std::cos(-angle);
-  const double edge = std::tan(angle) / radius;
-  ep.v[1][0] = edge;
-  ep.v[3][2] = -edge;
-  en.v[1][0] = -edge * -1;
-  en.v[3][2] = edge * -1;
-  for (const M4* e : {&bp, &ep, &d1, &en, &bn, &d2, &bn, &en, &d1, &ep, &bp, &d3}) lmul(m, *e);
+  const double rho = lb / std::sin(angle);
+  const M4 bend = sector_bend(angle, rho);
+  const M4 edge = edge_kick(std::tan(angle) / rho);
+  const M4 outer = drift(ld / std::cos(angle));  // dipole 1 to 2 and 3 to 4
+  const M4 centre = drift(lt - 4 * lb - 2 * ld);
+  const M4 back = drift(-lt);
+  M4 m = eye4();
+  for (const M4* e : {&bend, &edge, &outer, &edge, &bend, &centre, &bend, &edge, &outer, &edge,
+                      &bend, &back})
+    lmul(m, *e);
   return m;
 }
 void plane(double str, double off, int& mode, double& s, double& o) {

@@ -13,9 +13,16 @@
 // host from the same make_trans / make_push / make_dep as every other path,
 // for every step at creation, and kept on the device ([step][run]).
 //
-// Each group has its own context (its own stream), so the groups' launches
+// Each group has its own context (its own stream) and each step is one
+// particle launch and one solver step per group, so the groups' launches
 // overlap on the device; g4h_batch_advance joins them into the caller's
-// stream before it returns.
+// stream before it returns.  ADI batches can instead run every step of
+// every run in ONE launch (G4H_TUNE_BATCH = 1; launch_batch_persistent: one
+// workgroup per run loops over the steps, the per-step Thomas factors of each
+// group and the record index of each step in device arrays built here).  On
+// the taper scan that launch is slower than the per-step launches (the
+// sweeps' recurrences of a plane run on one CU with their memory latency
+// exposed), so it is not the default.
 //
 // Steady state only: one slice per run, no slippage (Gencore.cpp:229 is a
 // no-op there, ControlCUDA.cpp:150).
@@ -69,8 +76,22 @@ struct g4h_batch {
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // run -> (group, slice in group)
   int done = 0, nrec = 0, cap = 0;
+  std::vector<int32_t> rec_of_step;  // record index written by each step, or -1
+  // one-launch path
+  bool persistent = false;
+  int win_cells = 0, gmax = 0;
+  DevBuf<BatchRunDesc> druns;
+  DevBuf<int32_t> drec;
+  std::vector<DevBuf<const double2*>> dadi;  // per group: factors of each step
+  std::vector<DevBuf<double>> dcs;           // per group: coefficient of each step
+  hipEvent_t pdone = nullptr;
 
   ~g4h_batch() {
+    druns.release();
+    drec.release();
+    dadi.clear();
+    dcs.clear();
+    if (pdone) (void)hipEventDestroy(pdone);
     for (Group& g : groups) {
       if (g.field) g4h_field_destroy(g.field);
       if (g.beam) g4h_beam_destroy(g.beam);
@@ -106,7 +127,8 @@ struct g4h_batch {
   // once, so that a step enqueues launches only.
   void build_tables() {
     const int ns = static_cast<int>(steps.size());
-    std::vector<int> rec(static_cast<size_t>(ns), -1);
+    std::vector<int32_t>& rec = rec_of_step;
+    rec.assign(static_cast<size_t>(ns), -1);
     int r = 1;
     for (int k = 0; k < ns; ++k)
       if (steps[static_cast<size_t>(k)].outstep && r < cap) rec[static_cast<size_t>(k)] = r++;
@@ -151,6 +173,108 @@ struct g4h_batch {
       G4_HIP(hipMemcpy(g.dtab.get(), tab.data(), sizeof(ParticleParams) * tab.size(),
                        hipMemcpyHostToDevice));
     }
+  }
+
+  // The one-launch path: per-group per-step factor tables (every distinct
+  // step length's table stays resident: at most Field::kMaxDelzTables), run
+  // descriptors and record indices on the device.  Returns false (and the
+  // batch keeps the per-step launches) when the kernel does not fit.
+  bool setup_persistent(Context* c) {
+    if (solver != G4H_SOLVER_ADI) return false;
+    std::vector<double> lens;
+    for (const g4h_step& st : steps) lens.push_back(st.delz);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    if (lens.size() > Field::kMaxDelzTables) return false;
+    int wc = 0, gm = 0;
+    for (const Group& g : groups) {
+      gm = std::max(gm, g.G);
+      for (const ParticleParams& p : g.head) wc = std::max(wc, p.dep.window_cap);
+    }
+    if (!batch_persistent_fits(c, prec, nharm, wc, gm)) return false;
+    win_cells = wc;
+    gmax = gm;
+    const int ns = static_cast<int>(steps.size());
+    std::vector<BatchRunDesc> desc(where.size());
+    dadi.resize(groups.size());
+    dcs.resize(groups.size());
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      Group& g = groups[gi];
+      Field* f = g.field;
+      std::vector<const double2*> fac(static_cast<size_t>(ns));
+      std::vector<double> cs(static_cast<size_t>(ns));
+      for (int k = 0; k < ns; ++k) {
+        const double dz = steps[static_cast<size_t>(k)].delz;
+        field_prepare_solver(f, dz);
+        fac[static_cast<size_t>(k)] = f->adi_c;  // c, cbet, cwet contiguous
+        cs[static_cast<size_t>(k)] = 0.25 * dz / (f->xks * f->dgrid * f->dgrid);
+      }
+      if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
+      ensure_source(f);
+      dadi[gi].alloc(g.gctx, static_cast<size_t>(ns));
+      dcs[gi].alloc(g.gctx, static_cast<size_t>(ns));
+      G4_HIP(hipMemcpyAsync(dadi[gi].get(), fac.data(), sizeof(const double2*) * ns,
+                            hipMemcpyHostToDevice, g.gctx->stream));
+      G4_HIP(hipMemcpyAsync(dcs[gi].get(), cs.data(), sizeof(double) * ns, hipMemcpyHostToDevice,
+                            g.gctx->stream));
+      G4_HIP(hipStreamSynchronize(g.gctx->stream));
+      Beam* bm = g.beam;
+      for (size_t i = 0; i < g.runs.size(); ++i) {
+        BatchRunDesc d{};
+        if (prec == G4H_FP64) {
+          d.x = bm->xd.get();
+          d.y = bm->yd.get();
+          d.px = bm->pxd.get();
+          d.py = bm->pyd.get();
+          d.g = bm->gd.get();
+        } else {
+          d.x = bm->xf.get();
+          d.y = bm->yf.get();
+          d.px = bm->pxf.get();
+          d.py = bm->pyf.get();
+          d.g = bm->gf.get();
+        }
+        d.th = bm->theta.get();
+        d.off = bm->off.get();
+        d.tab = g.dtab.get();
+        d.u = f->data();
+        d.ys = f->adi_y.get();
+        d.src = f->src();
+        d.rect = f->src_rect.get();
+        d.adi = dadi[gi].get();
+        d.cs = dcs[gi].get();
+        d.field_hist = g.field_hist.get();
+        d.slice = static_cast<int>(i);
+        d.nrun = static_cast<int>(g.runs.size());
+        d.G = g.G;
+        desc[static_cast<size_t>(g.runs[i])] = d;
+      }
+    }
+    druns.alloc(c, desc.size());
+    drec.alloc(c, rec_of_step.size());
+    G4_HIP(hipMemcpyAsync(druns.get(), desc.data(), sizeof(BatchRunDesc) * desc.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipMemcpyAsync(drec.get(), rec_of_step.data(), sizeof(int32_t) * rec_of_step.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    G4_HIP(hipEventCreateWithFlags(&pdone, hipEventDisableTiming));
+    persistent = true;
+    return true;
+  }
+
+  // Steps [k0, k1) of every run in one launch on the batch's stream, after
+  // and before the groups' own streams.
+  void advance_persistent(int k0, int k1) {
+    for (Group& g : groups) {
+      G4_HIP(hipEventRecord(g.tail, g.gctx->stream));
+      G4_HIP(hipStreamWaitEvent(ctx->stream, g.tail, 0));
+    }
+    launch_batch_persistent(ctx, prec, nharm, druns.get(), static_cast<int>(druns.size()),
+                            drec.get(), k0, k1, win_cells, gmax);
+    G4_HIP(hipEventRecord(pdone, ctx->stream));
+    for (Group& g : groups) G4_HIP(hipStreamWaitEvent(g.gctx->stream, pdone, 0));
+    for (int k = k0; k < k1; ++k)
+      if (rec_of_step[static_cast<size_t>(k)] >= 0) ++nrec;
   }
 
   void step(int istep) {
@@ -268,6 +392,7 @@ g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solve
   }
   b->build_tables();
   b->record_initial();
+  if (ctx->tuner.forced[TUNE_BATCH] == 1) b->setup_persistent(ctx);
   *out = b.release();
   context_child_added(ctx);
   G4_API_END
@@ -289,6 +414,10 @@ g4h_status g4h_batch_advance(g4h_batch* b, int nsteps, int* done) {
   require(nsteps >= 0, "nsteps must be >= 0");
   b->ctx->activate();
   const int end = std::min<int>(static_cast<int>(b->steps.size()), b->done + nsteps);
+  if (b->persistent) {
+    if (end > b->done) b->advance_persistent(b->done, end);
+    b->done = end;
+  }
   for (; b->done < end; ++b->done) b->step(b->done);
   // the context's stream waits for every group, so work enqueued on it later
   // (or a synchronisation of it) comes after this advance

@@ -387,7 +387,7 @@ void write_dump(Deck& d, RunState& rs, const std::string& out, int gstep, double
 
 void write_records(const Deck& d, const std::string& out, const std::vector<double>& zs,
                    const std::vector<double>& braw, const std::vector<double>& fraw, int nrec,
-                   int nharm, const std::vector<g4h_step>& steps) {
+                   int nharm, bool exclharm, const std::vector<g4h_step>& steps) {
   const int S = d.nslice;
   const int bq = G4H_BEAM_QN(nharm);
   std::vector<double> q(static_cast<size_t>(nrec) * S * bq), bg(static_cast<size_t>(nrec) * G4H_BEAM_GLOBAL);
@@ -427,7 +427,10 @@ void write_records(const Deck& d, const std::string& out, const std::vector<doub
   };
   for (int c = 0; c < G4H_BEAM_Q; ++c)
     write_npy(out + "/Beam/" + bnames[c] + ".npy", column(q, bq, c), nrec, S);
+  // &track exclusive_harmonics: the fundamental and the highest harmonic
+  // only (DiagnosticCUDA.cpp:495-507)
   for (int h = 0; h < nharm; ++h) {
+    if (exclharm && h > 0 && h < nharm - 1) continue;
     const std::string suf = h == 0 ? "" : std::to_string(h + 1);
     write_npy(out + "/Beam/bunching" + suf + ".npy", column(q, bq, G4H_BEAM_Q + 2 * h), nrec, S);
     write_npy(out + "/Beam/bunchingphase" + suf + ".npy", column(q, bq, G4H_BEAM_Q + 2 * h + 1),
@@ -490,8 +493,7 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   if (field_dump_step < 0 || beam_dump_step < 0)
     throw std::runtime_error("&track dump steps must be >= 0");
   if (k.i("sort_step", 0) != 0) throw Unsupported("&track: sorting is not supported");
-  if (k.b("exclusive_harmonics", false))
-    throw Unsupported("&track exclusive_harmonics is not supported");
+  const bool exclharm = k.b("exclusive_harmonics", false);
   k.get("s0");
   k.get("slen");
   k.finish();
@@ -559,7 +561,7 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   d.zdone = steps.back().z;
   d.stepsdone += ns;
   const std::string dir = out + (d.nrecords > 0 ? "/track" + std::to_string(d.nrecords) : "");
-  write_records(d, dir, zs, braw, fraw, nrec, nharm, steps);
+  write_records(d, dir, zs, braw, fraw, nrec, nharm, exclharm, steps);
   d.nrecords += nrec;
 }
 

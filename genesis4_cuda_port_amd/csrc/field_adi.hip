@@ -19,7 +19,7 @@
 //    plane when asked.
 // The RHS, the source add and the Thomas recurrences are FP64 in both
 // precisions; the stored field is FP32 in MIXED.
-#include "kernels.hpp"
+#include "adi_lines.hpp"
 
 #include <complex>
 #include <vector>
@@ -27,40 +27,20 @@
 namespace g4 {
 namespace {
 
+using adi::A2;
+using adi::kLane;
+using adi::rhs_of;
+using adi::to_d;
+using adi::zadd;
+using adi::zmul;
+using adi::zsub;
+
 constexpr int kB = 8;   // columns per x-sweep block
 constexpr int kR = 8;   // rows per y-sweep chunk
-
-template <class P>
-struct A2;
-template <>
-struct A2<double> {
-  using T = double2;
-};
-template <>
-struct A2<float> {
-  using T = float2;
-};
 
 // Floating-point contraction is off in this file: the fused multiply-adds
 // below are the only ones, so every kernel variant rounds identically.
 #pragma clang fp contract(off)
-
-__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
-  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-
-// rhs = center + i*cs*((a + b) - 2 center) [+ s]
-__device__ __forceinline__ double2 rhs_of(double2 center, double2 a, double2 b, double cs) {
-  const double2 lap = zsub(zadd(a, b), make_double2(center.x * 2.0, center.y * 2.0));
-  return make_double2(fma(-cs, lap.y, center.x), fma(cs, lap.x, center.y));
-}
-
-template <class T2>
-__device__ __forceinline__ double2 to_d(T2 v) {
-  return make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
-}
 
 template <class P>
 __global__ __launch_bounds__(1024) void adi_x_kernel(typename A2<P>::T* u, double2* ys, int nslice,
@@ -300,55 +280,18 @@ __global__ __launch_bounds__(1024) void adi_y_kernel(typename A2<P>::T* u, doubl
 // For few planes (steady-state batches, short time windows) the workgroup
 // per plane above leaves most of the GPU idle and pays a barrier per row.
 // Here one 64-lane wave owns 64 rows (x sweep) or 64 columns (y sweep) of a
-// plane, so a plane of G = 255 is four independent waves.  The Laplacian
-// neighbours across lanes come from DPP wave shifts; the two edge lanes of
-// the wave load the row / column beyond the block themselves.  Every
-// element goes through the same operations in the same order as in the
-// workgroup kernels, so both variants give bit-identical fields and, with
-// the per-block partial sums reduced in block order, bit-identical stats.
+// plane (adi_lines.hpp), so a plane of G = 255 is four independent waves.
+// The blocks of a plane may run in any order and read each other's edge
+// lines, so the x sweep writes its result to a second plane set (ustar)
+// instead of in place, and the y sweep reads it from there.  Every element
+// goes through the same operations in the same order as in the workgroup
+// kernels, so both variants give bit-identical fields and, with the
+// per-block partial sums reduced in block order, bit-identical stats.
 
-constexpr int kLane = 64;
-
-// c, cbet, cwet of the step into LDS ([3][G]) by the one-wave workgroup
-__device__ __forceinline__ void stage_factors(double2* coef, const double2* c_d,
-                                              const double2* cbet_d, const double2* cwet_d,
-                                              int G) {
-  for (int k = threadIdx.x; k < G; k += blockDim.x) {
-    coef[k] = c_d[k];
-    coef[G + k] = cbet_d[k];
-    coef[2 * G + k] = cwet_d[k];
-  }
-  __syncthreads();
-}
-
-// lane i <- lane i-1 (wave_shr:1); lane 0 keeps `edge`
-__device__ __forceinline__ double2 from_lane_below(double2 v, double2 edge) {
-  int r[4];
-  const int* a = reinterpret_cast<const int*>(&v);
-  const int* e = reinterpret_cast<const int*>(&edge);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_update_dpp(e[k], a[k], 0x138, 0xf, 0xf, false);
-  double2 o;
-  __builtin_memcpy(&o, r, sizeof(o));
-  return o;
-}
-// lane i <- lane i+1 (wave_shl:1); lane 63 keeps `edge`
-__device__ __forceinline__ double2 from_lane_above(double2 v, double2 edge) {
-  int r[4];
-  const int* a = reinterpret_cast<const int*>(&v);
-  const int* e = reinterpret_cast<const int*>(&edge);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_update_dpp(e[k], a[k], 0x130, 0xf, 0xf, false);
-  double2 o;
-  __builtin_memcpy(&o, r, sizeof(o));
-  return o;
-}
-
-// x sweep: lane t of block b owns row iy = 64 b + t and walks its columns,
-// kCh elements per lane per chunk.
 template <class P, int kCh>
-__global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u, double2* ys,
-                                                           int nslice, int G, double cs,
+__global__ __launch_bounds__(kLane) void adi_x_lane_kernel(const typename A2<P>::T* u, double2* ys,
+                                                           typename A2<P>::T* ustar, int nslice,
+                                                           int G, double cs,
                                                            const double2* __restrict__ c_d,
                                                            const double2* __restrict__ cbet_d,
                                                            const double2* __restrict__ cwet_d,
@@ -361,18 +304,9 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
   // the Thomas factors in LDS: read in the recurrence at one element per
   // step, from memory they would put a load latency on every step
   extern __shared__ double2 coef[];
-  stage_factors(coef, c_d, cbet_d, cwet_d, G);
-  const double2* cl = coef;
-  const double2* cb = coef + G;
-  const double2* cw = coef + 2 * G;
-  const int iy = blockIdx.y * kLane + t;
-  const bool act = iy < G;
-  // the row beyond the block, loaded by the wave's edge lanes
-  const int ey = t == 0 ? iy - 1 : (t == kLane - 1 ? iy + 1 : -1);
-  const bool eact = ey >= 0 && ey < G;
+  const adi::Factors f = adi::stage_factors(coef, c_d, cbet_d, cwet_d, G, t, kLane);
+  __syncthreads();
   const int64_t plane = static_cast<int64_t>(G) * G;
-  T2* up = u + p * plane;
-  double2* yp = ys + p * plane;
   const int s = ((p - first) % nslice + nslice) % nslice;
   int4 rc = make_int4(0, 0, 0, 0);
   const T2* sp = nullptr;
@@ -380,95 +314,16 @@ __global__ __launch_bounds__(kLane) void adi_x_lane_kernel(typename A2<P>::T* u,
     rc = rect[s];
     sp = src + s * plane;
   }
-  const bool srow = sp && act && iy >= rc.y && iy < rc.w;
-  const T2* row = up + static_cast<int64_t>(act ? iy : 0) * G;
-  const T2* erow = up + static_cast<int64_t>(eact ? ey : 0) * G;
-  double2* yrow = yp + static_cast<int64_t>(act ? iy : 0) * G;
-  const double2 zero = make_double2(0.0, 0.0);
-  T2 tz;
-  tz.x = 0;
-  tz.y = 0;
-  // chunk loads run one chunk ahead of the recurrence
-  T2 nc[kCh], ne[kCh], ns[kCh];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int c = 0; c < kCh; ++c) {
-      const int k = k0 + c;
-      nc[c] = tz;
-      ne[c] = tz;
-      ns[c] = tz;
-      if (k < G) {
-        if (act) nc[c] = row[k];
-        if (eact) ne[c] = erow[k];
-        if (srow && k >= rc.x && k < rc.z) ns[c] = sp[static_cast<int64_t>(iy) * G + k];
-      }
-    }
-  };
-  load(0);
-  double2 yprev = zero;
-  for (int k0 = 0; k0 < G; k0 += kCh) {
-    double2 cen[kCh], edg[kCh], sv[kCh];
-#pragma unroll
-    for (int c = 0; c < kCh; ++c) {
-      cen[c] = to_d(nc[c]);
-      edg[c] = to_d(ne[c]);
-      sv[c] = to_d(ns[c]);
-    }
-    if (k0 + kCh < G) load(k0 + kCh);
-#pragma unroll
-    for (int c = 0; c < kCh; ++c) {
-      const int k = k0 + c;
-      if (k < G) {  // uniform
-        const double2 above = from_lane_below(cen[c], edg[c]);  // row iy - 1
-        const double2 below = from_lane_above(cen[c], edg[c]);  // row iy + 1
-        double2 r = rhs_of(cen[c], below, above, cs);
-        if (srow && k >= rc.x && k < rc.z) r = zadd(r, sv[c]);
-        yprev = zmul(zsub(r, zmul(cl[k], yprev)), cb[k]);
-        if (act) yrow[k] = yprev;
-      }
-    }
-  }
-  // backward substitution, right to left
-  double2 unext = zero;
-  T2* orow = up + static_cast<int64_t>(act ? iy : 0) * G;
-  double2 ny[kCh];
-  auto load_y = [&](int k0) {
-#pragma unroll
-    for (int c = 0; c < kCh; ++c) {
-      const int k = k0 + c;
-      ny[c] = zero;
-      if (act && k < G) ny[c] = yrow[k];
-    }
-  };
-  load_y(((G - 1) / kCh) * kCh);
-  for (int k0 = ((G - 1) / kCh) * kCh; k0 >= 0; k0 -= kCh) {
-    double2 y[kCh];
-#pragma unroll
-    for (int c = 0; c < kCh; ++c) y[c] = ny[c];
-    if (k0 > 0) load_y(k0 - kCh);
-#pragma unroll
-    for (int c = kCh - 1; c >= 0; --c) {
-      const int k = k0 + c;
-      if (k < G) {
-        const double2 uu = (k == G - 1) ? y[c] : zsub(y[c], zmul(cw[k + 1], unext));
-        unext = uu;
-        if (act) {
-          T2 o;
-          o.x = static_cast<P>(uu.x);
-          o.y = static_cast<P>(uu.y);
-          orow[k] = o;
-        }
-      }
-    }
-  }
+  adi::x_forward<P, kCh>(u + p * plane, ys + p * plane, G, cs, f, sp, rc, blockIdx.y, t);
+  adi::x_backward<P, kCh>(ys + p * plane, ustar + p * plane, G, f, blockIdx.y, t);
 }
 
-// y sweep: lane t of block b owns column ix = 64 b + t and walks its rows;
-// the backward pass writes the field and the block's partial field sums
-// (partials[(p * nblk + b) * 8 + k], centre cell at the tail), reduced in
-// block order afterwards.
+// y sweep of ustar into u; the back substitution writes the block's partial
+// field sums (partials[(p * nblk + b) * 8 + k], centre cell at the tail),
+// reduced in block order afterwards.
 template <class P, int kCh>
 __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u, double2* ys,
+                                                           const typename A2<P>::T* ustar,
                                                            int nslice, int G, double cs,
                                                            const double2* __restrict__ c_d,
                                                            const double2* __restrict__ cbet_d,
@@ -481,17 +336,9 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
   const int t = threadIdx.x;
   const int nblk = gridDim.y;
   extern __shared__ double2 coef[];
-  stage_factors(coef, c_d, cbet_d, cwet_d, G);
-  const double2* cl = coef;
-  const double2* cb = coef + G;
-  const double2* cw = coef + 2 * G;
-  const int ix = blockIdx.y * kLane + t;
-  const bool act = ix < G;
-  const int ex = t == 0 ? ix - 1 : (t == kLane - 1 ? ix + 1 : -1);
-  const bool eact = ex >= 0 && ex < G;
+  const adi::Factors f = adi::stage_factors(coef, c_d, cbet_d, cwet_d, G, t, kLane);
+  __syncthreads();
   const int64_t plane = static_cast<int64_t>(G) * G;
-  T2* up = u + p * plane;
-  double2* yp = ys + p * plane;
   const int s = ((p - first) % nslice + nslice) % nslice;
   int4 rc = make_int4(0, 0, 0, 0);
   const T2* sp = nullptr;
@@ -499,111 +346,34 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
     rc = rect[s];
     sp = src + s * plane;
   }
-  const bool scol = sp && act && ix >= rc.x && ix < rc.z;
-  const int cx = act ? ix : 0, cex = eact ? ex : 0;
-  const double2 zero = make_double2(0.0, 0.0);
-  T2 tz;
-  tz.x = 0;
-  tz.y = 0;
-  T2 nv[kCh], ne[kCh], ns[kCh];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int r = 0; r < kCh; ++r) {
-      const int k = k0 + r;
-      nv[r] = tz;
-      ne[r] = tz;
-      ns[r] = tz;
-      if (k < G) {
-        const int64_t rowoff = static_cast<int64_t>(k) * G;
-        if (act) nv[r] = ld_stream(&up[rowoff + cx]);
-        if (eact) ne[r] = up[rowoff + cex];
-        if (scol && k >= rc.y && k < rc.w) ns[r] = sp[rowoff + cx];
-      }
-    }
-  };
-  load(0);
-  double2 yprev = zero;
-  for (int k0 = 0; k0 < G; k0 += kCh) {
-    double2 v[kCh], e[kCh], sv[kCh];
-#pragma unroll
-    for (int r = 0; r < kCh; ++r) {
-      v[r] = to_d(nv[r]);
-      e[r] = to_d(ne[r]);
-      sv[r] = to_d(ns[r]);
-    }
-    if (k0 + kCh < G) load(k0 + kCh);
-#pragma unroll
-    for (int r = 0; r < kCh; ++r) {
-      const int k = k0 + r;
-      if (k < G) {  // uniform
-        const double2 left = from_lane_below(v[r], e[r]);   // column ix - 1
-        const double2 right = from_lane_above(v[r], e[r]);  // column ix + 1
-        double2 rr = rhs_of(v[r], right, left, cs);
-        if (scol && k >= rc.y && k < rc.w) rr = zadd(rr, sv[r]);
-        yprev = zmul(zsub(rr, zmul(cl[k], yprev)), cb[k]);
-        if (act) st_stream(&yp[static_cast<int64_t>(k) * G + cx], yprev);
-      }
-    }
-  }
-  const double shift = -0.5 * static_cast<double>(G - 1);
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  const int64_t icenter = (plane - 1) / 2;
-  double2 unext = zero;
-  double2 ny[kCh];
-  auto load_y = [&](int k0) {
-#pragma unroll
-    for (int r = 0; r < kCh; ++r) {
-      const int k = k0 + r;
-      ny[r] = zero;
-      if (act && k < G) ny[r] = ld_stream(&yp[static_cast<int64_t>(k) * G + cx]);
-    }
-  };
-  load_y(((G - 1) / kCh) * kCh);
-  for (int k0 = ((G - 1) / kCh) * kCh; k0 >= 0; k0 -= kCh) {
-    double2 y[kCh];
-#pragma unroll
-    for (int r = 0; r < kCh; ++r) y[r] = ny[r];
-    if (k0 > 0) load_y(k0 - kCh);
-#pragma unroll
-    for (int r = kCh - 1; r >= 0; --r) {
-      const int k = k0 + r;
-      if (act && k < G) {
-        const double2 uu = (k == G - 1) ? y[r] : zsub(y[r], zmul(cw[k + 1], unext));
-        unext = uu;
-        T2 o;
-        o.x = static_cast<P>(uu.x);
-        o.y = static_cast<P>(uu.y);
-        const int64_t gi = static_cast<int64_t>(k) * G + cx;
-        st_stream(&up[gi], o);
-        if (partials) {
-          const double re = o.x, im = o.y;
-          const double wei = re * re + im * im;
-          const double dx = ix + shift, dy = k + shift;
-          acc[0] += wei;
-          acc[1] += dx * wei;
-          acc[2] += dx * dx * wei;
-          acc[3] += dy * wei;
-          acc[4] += dy * dy * wei;
-          acc[5] += re;
-          acc[6] += im;
-          if (gi == icenter) {
-            double* c = partials + static_cast<int64_t>(nslice) * nblk * 8 + 2 * p;
-            c[0] = re;
-            c[1] = im;
-          }
-        }
-      }
-    }
-  }
+  adi::y_forward<P, kCh>(ustar + p * plane, ys + p * plane, G, cs, f, sp, rc, blockIdx.y, t);
+  adi::PlaneSums sums;
+  sums.clear();
   if (partials) {
+    double* centre = partials + static_cast<int64_t>(nslice) * nblk * 8 + 2 * p;
+    adi::y_backward<P, kCh, true>(ys + p * plane, u + p * plane, G, f, blockIdx.y, t, sums,
+                                  centre);
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      double v = acc[k];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      const double v = adi::wave_sum64(sums.acc[k]);
       if (t == 0) partials[(static_cast<int64_t>(p) * nblk + blockIdx.y) * 8 + k] = v;
     }
+  } else {
+    adi::y_backward<P, kCh, false>(ys + p * plane, u + p * plane, G, f, blockIdx.y, t, sums,
+                                   nullptr);
   }
+}
+
+// The x sweep's output of the wave-per-block variants (the y sweep's input),
+// one plane per ring slot in the field's storage type.
+void* adi_ustar(Field* f) {
+  const size_t n = static_cast<size_t>(f->cells());
+  if (f->prec == G4H_FP64) {
+    if (f->adi_sd.size() < n) f->adi_sd.alloc(f->ctx, n);
+    return f->adi_sd.get();
+  }
+  if (f->adi_sf.size() < n) f->adi_sf.alloc(f->ctx, n);
+  return f->adi_sf.get();
 }
 
 template <class P>
@@ -635,16 +405,17 @@ void adi_step(Field* f, bool source_on, bool stats) {
       partials = f->col_partials.get();
     }
     const size_t ldsc = sizeof(double2) * 3 * static_cast<size_t>(G);
+    T2* ustar = static_cast<T2*>(adi_ustar(f));
     auto sweeps = [&](auto ch) {
       constexpr int CH = decltype(ch)::value;
       f->ctx->launch(G4H_K_ADI_X, [&] {
         hipLaunchKernelGGL((adi_x_lane_kernel<P, CH>), dim3(f->nslice, nblk), dim3(kLane), ldsc,
-                           f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                           f->ctx->stream, u, f->adi_y.get(), ustar, f->nslice, G, cs, f->adi_c,
                            f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add);
       });
       f->ctx->launch(G4H_K_ADI_Y, [&] {
         hipLaunchKernelGGL((adi_y_lane_kernel<P, CH>), dim3(f->nslice, nblk), dim3(kLane), ldsc,
-                           f->ctx->stream, u, f->adi_y.get(), f->nslice, G, cs, f->adi_c,
+                           f->ctx->stream, u, f->adi_y.get(), ustar, f->nslice, G, cs, f->adi_c,
                            f->adi_cbet, f->adi_cwet, src, f->src_rect.get(), f->first, add,
                            partials);
       });

@@ -304,15 +304,15 @@ struct RegCfg {
   static constexpr int kThreads = 64 * kWaves;
   // column tile: one column per line slot of the workgroup
   static constexpr int kTile = kWaves * S::LPW < G ? kWaves * S::LPW : G;
-  // the column pass stages its tile (G rows of kTile + 1 padded columns)
+  // the column pass stages its tile (G rows of kTile columns, swizzled)
   // through LDS when every line slot holds a column, a tile row is at least
-  // 64 bytes and the staged copy covers the G rows in whole instructions
+  // 64 bytes, kTile is a power of two and the staged copy covers the G rows
+  // in whole instructions
   static constexpr bool kStage = kWaves * S::LPW <= G && kThreads % kTile == 0 &&
-                                 G % (kThreads / kTile) == 0 &&
+                                 G % (kThreads / kTile) == 0 && (kTile & (kTile - 1)) == 0 &&
                                  kTile * 2 * static_cast<int>(sizeof(P)) >= 64;
   static constexpr int kBuf = kWaves * S::LPW * S::BUF;
-  static constexpr int kColBuf =
-      kStage && G * (kTile + 1) > kBuf ? G * (kTile + 1) : kBuf;
+  static constexpr int kColBuf = kStage && G * kTile > kBuf ? G * kTile : kBuf;
 };
 
 template <class T2>
@@ -423,8 +423,14 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   // Staged tile: the workgroup's kTile columns of all G rows pass through
   // LDS (reusing the transpose buffer), so each load and store instruction
   // covers whole row segments of the tile instead of L rows x LPW columns.
-  constexpr int TW = C::kTile, TP = TW + 1;
+  // Cell (row, col) of the tile sits at row * TW + (col ^ (row % TW)): the
+  // row-segment copies write and read whole rows (any order within a row is
+  // conflict-free), and the line slots' reads of one column down L rows hit
+  // TW different positions, where a plain or +1-padded row would put
+  // several lanes on one LDS bank.
+  constexpr int TW = C::kTile;
   constexpr bool kStage = C::kStage;
+  auto swz = [](int row, int col) { return row * TW + (col ^ (row & (TW - 1))); };
   constexpr int RPI = C::kThreads / TW;  // rows per instruction of the staged copy
   const int c0 = tile * TW;
   T2 v[R];
@@ -433,12 +439,12 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     fftr::static_for<0, G / RPI>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const int row = i * RPI + sc_row;
-      buf[row * TP + sc_col] = ld_stream(&plane[static_cast<int64_t>(row) * G + c0 + sc_col]);
+      buf[swz(row, sc_col)] = ld_stream(&plane[static_cast<int64_t>(row) * G + c0 + sc_col]);
     });
     __syncthreads();
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
-      v[r] = buf[(t + L * r) * TP + cslot];
+      v[r] = buf[swz(t + L * r, cslot)];
     });
     __syncthreads();
   } else {
@@ -483,7 +489,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     __syncthreads();  // the inverse transform's LDS use is over
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
-      buf[(t + L * r) * TP + cslot] = v[r];
+      buf[swz(t + L * r, cslot)] = v[r];
     });
     __syncthreads();
     const int sc_col = threadIdx.x % TW, sc_row = threadIdx.x / TW;
@@ -492,7 +498,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     const double dx = col + shift;
     // the stored value of tile cell (row, this thread's column)
     auto out = [&](int row, int64_t gi) {
-      T2 o = buf[row * TP + sc_col];
+      T2 o = buf[swz(row, sc_col)];
       o.x *= sc;
       o.y *= sc;
       if (scol && row >= rc.y && row < rc.w) {

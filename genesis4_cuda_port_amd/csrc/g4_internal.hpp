@@ -108,6 +108,8 @@ enum TuneFamily : int {
   TUNE_ADI = 0,    // 0: workgroup per plane (LDS-staged), 1: one wave per 64 lines
   TUNE_SPLIT = 1,  // workgroups per slice of the particle pass: not tuned, chosen
                    // from the slice count; pinning sets it (>= 1)
+  TUNE_BATCH = 2,  // ADI batches: 0 one launch per kernel and step (default), 1 one
+                   // launch for all steps
   TUNE_COUNT
 };
 
@@ -128,7 +130,7 @@ struct Tuner {
     std::vector<Trial> trials;
   };
   std::map<std::vector<int>, Entry> table;
-  int forced[TUNE_COUNT] = {-1, -1};
+  int forced[TUNE_COUNT] = {-1, -1, -1};
 };
 
 struct Context {
@@ -338,6 +340,8 @@ struct Field {
   const double2* adi_cbet = nullptr;
   const double2* adi_cwet = nullptr;
   DevBuf<double2> adi_y;  // FP64 intermediates [nslice*ngrid^2]
+  DevBuf<double2> adi_sd;  // x-sweep output of the wave-per-block sweeps, FP64 planes
+  DevBuf<float2> adi_sf;   // same, MIXED planes
   // FFT
   double fft_delz = 0.0;
   DevBuf<double2> fft_tw;            // four-step twiddles / DFT twiddles, [G]

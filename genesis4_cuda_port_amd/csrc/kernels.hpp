@@ -116,6 +116,41 @@ struct ParticleParams {
 void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
                      const ParticleParams* d_table = nullptr);
 void launch_cell_index(Context* ctx, Beam* b, const Field* f, int32_t* d_cell);
+// One run of a persistent batch launch (launch_batch_persistent): where its
+// slice lives in its group's beam and field, and its group's per-step tables.
+struct BatchRunDesc {
+  void* x;  // group beam arrays (double* in FP64, float* in MIXED)
+  void* y;
+  void* px;
+  void* py;
+  void* g;
+  double* th;
+  const int32_t* off;          // group slice offsets
+  const ParticleParams* tab;   // group parameter table [step][nrun]
+  void* u;                     // group field planes (double2* / float2*)
+  double2* ys;                 // group FP64 forward-value planes
+  const void* src;             // group source planes
+  const int4* rect;            // group source rectangles
+  const double2* const* adi;   // per step: Thomas factors c, cbet, cwet [3G]
+  const double* cs;            // per step: the sweeps' i*rtmp coefficient
+  double* field_hist;          // group field records [record][nrun][G4H_FIELD_RAW]
+  int slice;                   // the run's slice (= plane) in its group
+  int nrun;                    // runs in the group
+  int G;
+  int pad_;
+};
+
+// Steps [k0, k1) of every run of a batch of steady-state ADI runs in ONE
+// launch: one workgroup per run loops over the steps (particle pass, both
+// sweeps, records on steps with rec[k] >= 0), with workgroup barriers only.
+// lds_bytes: window and factor space (max window cells + 3 max G) * 16.
+void launch_batch_persistent(Context* ctx, g4h_precision prec, int nharm,
+                             const BatchRunDesc* d_runs, int nruns, const int32_t* d_rec,
+                             int k0, int k1, int win_cells, int gmax);
+// Whether the batch kernel can run grids up to gmax with a window of
+// win_cells on the context's GPU (ngrid <= 512, LDS budget).
+bool batch_persistent_fits(Context* ctx, g4h_precision prec, int nharm, int win_cells, int gmax);
+
 // Beam sums into the library's fixed device array (no allocation): blocking,
 // raw[nslice * G4H_BEAM_RAW(nharm)] on the host; fails with
 // G4H_ERR_INVALID_ARGUMENT when the pass needs more than kNoallocScratchBytes.

@@ -43,13 +43,13 @@
 #include <string>
 #include <vector>
 
+#include "adi_lines.hpp"
 #include "kernels.hpp"
 
 namespace g4 {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 constexpr int kMaxHarm = 8;
 constexpr int kNMinMax = 10;
 
@@ -605,8 +605,6 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   base_r = r0[0];
 }
 
-__device__ __forceinline__ double gabs(double g, double) { return g; }
-__device__ __forceinline__ double gabs(float dg, double g0) { return g0 + double(dg); }
 __device__ __forceinline__ double gshift(double g, double g0) { return g - g0; }
 __device__ __forceinline__ double gshift(float dg, double) { return double(dg); }
 
@@ -722,7 +720,7 @@ struct SplitScratch {
 };
 
 template <class P, int NF, bool DEP, int NH>
-__device__ __forceinline__ void process(const ParticleParams& prm, const MixedK& mk, int s,
+__device__ __forceinline__ void process(bool diag_on, const ParticleParams& prm, const MixedK& mk, int s,
                                         int64_t i, BeamPtr<P> b, P x, P y, P px, P py, P g,
                                         double th, P scl, typename V2<P>::T* src, double2* win,
                                         int wx0, int wy0, int ww, int wh, int* s_fb,
@@ -856,6 +854,7 @@ __device__ __forceinline__ void process(const ParticleParams& prm, const MixedK&
     }
   }
   if constexpr (NH > 0) {
+    if (!diag_on) return;
     const double dg = gshift(g, g0);
     ps.e[0] += dg;
     ps.e[1] += dg * dg;
@@ -921,24 +920,29 @@ __device__ __forceinline__ const ParticleParams& slice_params(const ParticlePara
     return arg;
 }
 
-template <class P, int NF, bool DEP, int NH, bool TAB>
-__global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
-                                                            ParticleParams prm_arg,
-                                                            const ParticleParams* __restrict__ tab,
-                                                            SplitScratch sp) {
-  const ParticleParams& prm = slice_params<TAB, P>(prm_arg, tab);
+// The particle pass over part kp of K of slice s, by one workgroup of NT
+// threads (NT / 64 waves) whose dynamic LDS holds the deposition window win.
+//
+// With RT the deposition and the beam sums run only when prm.stages asks for
+// them (ST_DEP, ST_DIAG; uniform over the workgroup), so that one copy of the
+// pass serves every step of a persistent launch; otherwise DEP and NH > 0
+// decide at compile time.
+template <class P, int NF, bool DEP, int NH, int NT, bool RT = false>
+__device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
+                                               const ParticleParams& prm, SplitScratch sp, int s,
+                                               int K, int kp, double2* win) {
+  constexpr int NW = NT / 64;
+  const bool dep_on = DEP && (!RT || (prm.stages & ST_DEP) != 0);
+  const bool diag_on = NH > 0 && (!RT || (prm.stages & ST_DIAG) != 0);
   using T2 = typename V2<P>::T;
   constexpr int NSUM = NH > 0 ? 12 + 2 * NH : 1;  // sums other than the count
-  extern __shared__ double2 win[];
   __shared__ int s_bb[4];  // touched-cell bbox (min x, min y, max x, max y), inclusive
   __shared__ int s_fb[4];  // same, for cells reached through the global fallback
-  __shared__ double s_red[kWaves][NSUM + kNMinMax];
+  __shared__ double s_red[NW][NSUM + kNMinMax];
 
-  const int s = blockIdx.x;
   const int tid = threadIdx.x;
-  // With K = gridDim.y > 1 the slice's particles are split into K contiguous
-  // parts, one per workgroup; the last part to finish combines them.
-  const int K = gridDim.y, kp = blockIdx.y;
+  // With K > 1 the slice's particles are split into K contiguous parts, one
+  // per workgroup; the last part to finish combines them.
   const int64_t i0s = off[s], i1s = off[s + 1];
   int64_t i0 = i0s, i1 = i1s;
   if (K > 1) {
@@ -955,7 +959,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   int wx0 = 0, wy0 = 0, ww = 0, wh = 0;
   P scl = P(0);
   T2* src = nullptr;
-  if constexpr (DEP) {
+  if constexpr (DEP) if (dep_on) {
     src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
     scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
     int4 bb = prm.dep.bbox[s];
@@ -971,7 +975,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       }
       __syncthreads();
       int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
-      for (int64_t k = i0s + tid; k < i1s; k += kThreads) {
+      for (int64_t k = i0s + tid; k < i1s; k += NT) {
         int ix, iy;
         P wx, wy;
         if (cell_of<P>(b.x[k], b.y[k], P(prm.dep.gridmax), P(prm.dep.dgrid),
@@ -1015,7 +1019,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     const int4 old = prm.dep.rect[s];
     const int ow = old.z - old.x, oh = old.w - old.y;
     if (K == 1 && ow > 0 && oh > 0) {
-      for (int k = tid; k < ow * oh; k += kThreads) {
+      for (int k = tid; k < ow * oh; k += NT) {
         const int cy = old.y + k / ow, cx = old.x + k % ow;
         T2 z;
         z.x = 0;
@@ -1023,7 +1027,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
         src[cx + static_cast<int64_t>(cy) * ng] = z;
       }
     }
-    for (int k = tid; k < ww * wh; k += kThreads) win[k] = make_double2(0.0, 0.0);
+    for (int k = tid; k < ww * wh; k += NT) win[k] = make_double2(0.0, 0.0);
     if (tid == 0) {
       s_bb[0] = s_bb[1] = 0x7fffffff;
       s_bb[2] = s_bb[3] = -1;
@@ -1034,7 +1038,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   }
 
   PassState<P, NH> ps;
-  if constexpr (NH > 0) {
+  if constexpr (NH > 0) if (diag_on) {
 #pragma unroll
     for (int k = 0; k < 2 + 2 * NH; ++k) ps.e[k] = 0.0;
 #pragma unroll
@@ -1076,7 +1080,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
   while (have) {
     const P x = nx, y = ny, px = npx, py = npy, g = ngm;
     const double th = nth;
-    const int64_t inext = i + kThreads;
+    const int64_t inext = i + NT;
     const bool hnext = inext < i1;
     if (hnext) {
       nx = ld_stream(&b.x[inext]);
@@ -1086,7 +1090,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       ngm = ld_stream(&b.g[inext]);
       nth = ld_stream(&b.th[inext]);
     }
-    process<P, NF, DEP, NH>(prm, mk, s, i, b, x, y, px, py, g, th, scl, src, win, wx0, wy0, ww,
+    process<P, NF, DEP, NH>(diag_on, prm, mk, s, i, b, x, y, px, py, g, th, scl, src, win, wx0, wy0, ww,
                             wh, s_fb, ps);
     i = inext;
     have = hnext;
@@ -1094,7 +1098,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
 
   const int nharm = NH > 0 ? prm.diag.nharm : 0;
   const int nsum = 12 + 2 * nharm;
-  if constexpr (NH > 0) {
+  if constexpr (NH > 0) if (diag_on) {
     // per-wave sums and extremes
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -1117,21 +1121,21 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     // to arrive goes on to combine all K of them in part order
     __syncthreads();
     const int64_t part = static_cast<int64_t>(s) * K + kp;
-    if constexpr (DEP) {
+    if constexpr (DEP) if (dep_on) {
       double2* w = sp.win + part * prm.dep.window_cap;
-      for (int k = tid; k < ww * wh; k += kThreads) w[k] = win[k];
+      for (int k = tid; k < ww * wh; k += NT) w[k] = win[k];
       if (tid == 0) sp.fb[part] = make_int4(s_fb[0], s_fb[1], s_fb[2], s_fb[3]);
     }
-    if constexpr (NH > 0) {
+    if constexpr (NH > 0) if (diag_on) {
       double* r = sp.red + part * kRedStride;
       if (tid < nsum) {
         double v = 0.0;
-        for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+        for (int w = 0; w < NW; ++w) v += s_red[w][tid];
         r[tid] = v;
       } else if (tid >= 128 && tid < 128 + kNMinMax) {
         const int k = tid - 128;
         double v = s_red[0][NSUM + k];
-        for (int w = 1; w < kWaves; ++w)
+        for (int w = 1; w < NW; ++w)
           v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
         r[kSumMax + k] = v;
       }
@@ -1144,7 +1148,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     if (!s_last) return;
     __threadfence();
     if (tid == 0) {
-      if constexpr (DEP) {
+      if constexpr (DEP) if (dep_on) {
         for (int q = 0; q < K; ++q) {
           const int4 f = sp.fb[static_cast<int64_t>(s) * K + q];
           s_fb[0] = min(s_fb[0], f.x);
@@ -1158,13 +1162,13 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     __syncthreads();
   }
 
-  if constexpr (DEP) {
+  if constexpr (DEP) if (dep_on) {
     __syncthreads();
     // write the window with plain stores; its non-zero cells give the
     // footprint for the next step's window
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
     const double2* parts = sp.win + static_cast<int64_t>(s) * K * prm.dep.window_cap;
-    for (int k = tid; k < ww * wh; k += kThreads) {
+    for (int k = tid; k < ww * wh; k += NT) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
       double2 v = win[k];
       if (K > 1) {
@@ -1215,7 +1219,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     }
   }
 
-  if constexpr (NH > 0) {
+  if constexpr (NH > 0) if (diag_on) {
     __syncthreads();
     const double* parts = sp.red + static_cast<int64_t>(s) * K * kRedStride;
     double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
@@ -1224,7 +1228,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
     if (tid < nsum) {
       double v = 0.0;
       if (K == 1) {
-        for (int w = 0; w < kWaves; ++w) v += s_red[w][tid];
+        for (int w = 0; w < NW; ++w) v += s_red[w][tid];
       } else {
         for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
       }
@@ -1237,7 +1241,7 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       double v;
       if (K == 1) {
         v = s_red[0][NSUM + k];
-        for (int w = 1; w < kWaves; ++w)
+        for (int w = 1; w < NW; ++w)
           v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
       } else {
         v = parts[kSumMax + k];
@@ -1250,6 +1254,16 @@ __global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const 
       out[G4H_B_XMIN + k] = v;
     }
   }
+}
+
+template <class P, int NF, bool DEP, int NH, bool TAB>
+__global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
+                                                            ParticleParams prm_arg,
+                                                            const ParticleParams* __restrict__ tab,
+                                                            SplitScratch sp) {
+  extern __shared__ double2 win[];
+  particle_slice<P, NF, DEP, NH, kThreads>(b, off, slice_params<TAB, P>(prm_arg, tab), sp,
+                                           blockIdx.x, gridDim.y, blockIdx.y, win);
 }
 
 template <class P>
@@ -1413,6 +1427,103 @@ void dispatch(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
   return dispatch_nh<P, 0, false, false>(ctx, b, p, bp, nullptr);
 }
 
+// ---- batches of steady-state runs in one launch ----------------------------
+// One workgroup of kBatchThreads owns one run (one slice, one ring plane) and
+// loops over the steps: the fused particle pass over the run's particles
+// (the same particle_slice as the grid launch, unsplit), then the two ADI
+// sweeps of its plane with the wave-per-64-lines code of adi_lines.hpp
+// (waves take the plane's 64-line blocks in turn; a barrier between the
+// forward and the backward pass of each sweep makes the in-place update
+// safe), and on output steps the plane's field sums, reduced in block order
+// as the grid launch reduces them.  No grid-wide synchronisation: runs are
+// independent.  Dynamic LDS: the deposition window, then the factors.
+constexpr int kBatchThreads = 512;
+constexpr int kBatchWaves = kBatchThreads / 64;
+constexpr int kBatchChunk = 4;
+
+template <class P, int NH>
+__device__ __forceinline__ void batch_particles(const BatchRunDesc& R, const ParticleParams& prm,
+                                                double2* win) {
+  BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
+               static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
+  const SplitScratch none{nullptr, nullptr, nullptr, nullptr};
+  // one copy of the pass; deposition and sums as the step's stages ask
+  particle_slice<P, 1, true, NH, kBatchThreads, true>(b, R.off, prm, none, R.slice, 1, 0, win);
+}
+
+template <class P, int NH>
+__global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
+    const BatchRunDesc* __restrict__ runs, const int32_t* __restrict__ rec, int k0, int k1,
+    int win_cells) {
+  using T2 = typename V2<P>::T;
+  extern __shared__ double2 lds[];
+  __shared__ double s_part[kBatchWaves][8];
+  __shared__ double s_centre[2];
+  const BatchRunDesc R = runs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int G = R.G;
+  const int nblk = (G + adi::kLane - 1) / adi::kLane;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  T2* u = static_cast<T2*>(R.u) + R.slice * plane;
+  double2* ys = R.ys + R.slice * plane;
+  const T2* srcp = static_cast<const T2*>(R.src) + R.slice * plane;
+  double2* win = lds;
+  double2* coef = lds + win_cells;
+  for (int k = k0; k < k1; ++k) {
+    const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
+    batch_particles<P, NH>(R, prm, win);
+    const bool dep = (prm.stages & ST_DEP) != 0;
+    const double2* fac = R.adi[k];
+    const double cs = R.cs[k];
+    const int r = rec[k];
+    __syncthreads();  // source plane and rectangle written, window LDS free
+    const adi::Factors f = adi::stage_factors(coef, fac, fac + G, fac + 2 * G, G, tid,
+                                              kBatchThreads);
+    const T2* sp = dep ? srcp : nullptr;
+    const int4 rc = dep ? R.rect[R.slice] : make_int4(0, 0, 0, 0);
+    __syncthreads();
+    for (int blk = wave; blk < nblk; blk += kBatchWaves)
+      adi::x_forward<P, kBatchChunk>(u, ys, G, cs, f, sp, rc, blk, lane);
+    __syncthreads();  // every block has read its neighbours' rows
+    for (int blk = wave; blk < nblk; blk += kBatchWaves)
+      adi::x_backward<P, kBatchChunk>(ys, u, G, f, blk, lane);
+    __syncthreads();
+    for (int blk = wave; blk < nblk; blk += kBatchWaves)
+      adi::y_forward<P, kBatchChunk>(u, ys, G, cs, f, sp, rc, blk, lane);
+    __syncthreads();
+    if (r >= 0) {
+      for (int blk = wave; blk < nblk; blk += kBatchWaves) {
+        adi::PlaneSums sums;
+        sums.clear();
+        adi::y_backward<P, kBatchChunk, true>(ys, u, G, f, blk, lane, sums, s_centre);
+        // one wave per block here (nblk <= kBatchWaves for G <= 512)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          const double v = adi::wave_sum64(sums.acc[q]);
+          if (lane == 0) s_part[blk][q] = v;
+        }
+      }
+      __syncthreads();
+      if (tid < G4H_FIELD_RAW) {
+        double v;
+        if (tid < 7) {
+          v = 0.0;
+          for (int q = 0; q < nblk; ++q) v += s_part[q][tid];
+        } else {
+          v = s_centre[tid - 7];
+        }
+        R.field_hist[(static_cast<int64_t>(r) * R.nrun + R.slice) * G4H_FIELD_RAW + tid] = v;
+      }
+    } else {
+      adi::PlaneSums none;
+      for (int blk = wave; blk < nblk; blk += kBatchWaves)
+        adi::y_backward<P, kBatchChunk, false>(ys, u, G, f, blk, lane, none, nullptr);
+    }
+    __syncthreads();  // the plane is final before the next step gathers from it
+  }
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1488,6 +1599,55 @@ void launch_particle(Context* ctx, Beam* b, const ParticleParams& p,
                       b->theta.get()};
     dispatch<float>(ctx, b, p, bp, d_table);
   }
+}
+
+namespace {
+template <class F>
+void batch_kernel_of(g4h_precision prec, int nharm, F&& use) {
+  if (prec == G4H_FP64) {
+    if (nharm == 1)
+      use(batch_persistent_kernel<double, 1>);
+    else
+      use(batch_persistent_kernel<double, 8>);
+  } else {
+    if (nharm == 1)
+      use(batch_persistent_kernel<float, 1>);
+    else
+      use(batch_persistent_kernel<float, 8>);
+  }
+}
+
+size_t batch_lds_bytes(int win_cells, int gmax) {
+  return sizeof(double2) * (static_cast<size_t>(win_cells) + 3 * static_cast<size_t>(gmax));
+}
+}  // namespace
+
+bool batch_persistent_fits(Context* ctx, g4h_precision prec, int nharm, int win_cells, int gmax) {
+  if (gmax > kBatchWaves * adi::kLane || nharm < 1 || nharm > kMaxHarm) return false;
+  int limit = 0;
+  G4_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  size_t statics = 0;
+  batch_kernel_of(prec, nharm, [&](auto kern) {
+    hipFuncAttributes attr{};
+    G4_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
+    statics = attr.sharedSizeBytes;
+  });
+  return batch_lds_bytes(win_cells, gmax) + statics <= static_cast<size_t>(limit);
+}
+
+void launch_batch_persistent(Context* ctx, g4h_precision prec, int nharm,
+                             const BatchRunDesc* d_runs, int nruns, const int32_t* d_rec, int k0,
+                             int k1, int win_cells, int gmax) {
+  if (nruns <= 0 || k1 <= k0) return;
+  require(batch_persistent_fits(ctx, prec, nharm, win_cells, gmax),
+          "batch kernel: grid above 512 or LDS budget exceeded");
+  const size_t lds = batch_lds_bytes(win_cells, gmax);
+  batch_kernel_of(prec, nharm, [&](auto kern) {
+    ctx->launch(G4H_K_PARTICLE, [&] {
+      hipLaunchKernelGGL(kern, dim3(nruns), dim3(kBatchThreads), lds, ctx->stream, d_runs, d_rec,
+                         k0, k1, win_cells);
+    });
+  });
 }
 
 // ---- beam sums without allocation ----------------------------------------

@@ -55,7 +55,8 @@ struct Req {
 };
 
 // HBM estimate of a request's resident state (beam, field ring, source
-// planes, FP64 ADI intermediate, diagnostics history).
+// planes, FP64 ADI intermediate and the x sweep's output plane set,
+// diagnostics history).
 uint64_t estimate_bytes(const g4h_request& q, int nsteps) {
   const uint64_t np = static_cast<uint64_t>(q.nslice) * q.npart;
   const uint64_t cells = static_cast<uint64_t>(q.nslice) * q.ngrid * q.ngrid;
@@ -67,7 +68,7 @@ uint64_t estimate_bytes(const g4h_request& q, int nsteps) {
       static_cast<uint64_t>(1 + nout) * q.nslice * (G4H_BEAM_RAW(q.nharm) + G4H_FIELD_RAW) * 8;
   // a steady-state request in a batch also holds one parameter block per step
   const uint64_t tables = q.kind == 0 ? static_cast<uint64_t>(nsteps) * sizeof(g4::ParticleParams) : 0;
-  return np * sp + cells * (2 * sc + (q.solver == G4H_SOLVER_ADI ? 16 : 0)) + hist + tables +
+  return np * sp + cells * (2 * sc + (q.solver == G4H_SOLVER_ADI ? 16 + sc : 0)) + hist + tables +
          (1u << 20);
 }
 

@@ -6,7 +6,10 @@ aw taper) must give the records it gives when run alone through g4h_run
 and one run is checked against the FP64 oracle.
 
 Bars: FP64 1e-12 relative against the lone run and 1e-10 against the
-oracle on slice power and bunching; MIXED 1e-6 against the lone MIXED run."""
+oracle on slice power and bunching; MIXED 1e-6 against the lone MIXED run.
+ADI batches can also run as one launch over all steps (G4H_TUNE_BATCH = 1);
+that batch must agree with the default one launch per kernel and step to
+the same bars."""
 from __future__ import annotations
 
 import numpy as np
@@ -16,6 +19,7 @@ import _oracle
 from _oracle import OField
 from cases import AW, GAMMA0, LAMBDA0, LAMBDAU
 from genesis4_cuda_port_amd import api
+from genesis4_cuda_port_amd._abi import TUNE_BATCH
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +114,32 @@ def test_batch_runs_match_lone_runs(gpu_ctx, prec, solver):
         f.close()
         b.close()
     batch.close()
+
+
+@pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
+def test_batch_one_launch_matches_per_step_launches(prec):
+    print(f"case batch one launch vs per step prec={prec}", flush=True)
+    runs = _runs()
+    runs.append(dict(runs[0], npart=4096, ngrid=101, seed=16, seed_power=1e5))
+    runs.append(dict(runs[1], npart=2048, ngrid=255, seed=17))
+    steps = _steps()
+    out = {}
+    for variant in (0, 1):
+        with api.Context(0) as ctx:
+            ctx.set_variant(TUNE_BATCH, variant)
+            batch = api.Batch(ctx, prec, api.ADI, runs, steps, LAMBDA0)
+            # in two pieces: the one-launch path resumes where it stopped
+            assert batch.advance(NSTEPS // 2) == NSTEPS // 2
+            assert batch.advance(NSTEPS) == NSTEPS
+            assert batch.records() == NSTEPS + 1
+            out[variant] = [batch.download(i) for i in range(len(runs))]
+            batch.close()
+    tol = 1e-12 if prec == api.FP64 else 1e-6
+    for i, r in enumerate(runs):
+        (b0, f0), (b1, f1) = out[0][i], out[1][i]
+        P0, B0 = _power_bunching(b0[:, None, :], f0[:, None, :], r["ngrid"], r["current"])
+        P1, B1 = _power_bunching(b1[:, None, :], f1[:, None, :], r["ngrid"], r["current"])
+        ep, eb = relerr(P1, P0), relerr(B1, B0)
+        print(f"  run {i} (ngrid {r['ngrid']}): power {ep:.3e} bunching {eb:.3e}", flush=True)
+        assert ep < tol and eb < tol, i
+        assert np.array_equal(b0[:, 12], b1[:, 12])  # particle counts

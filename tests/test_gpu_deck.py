@@ -123,3 +123,23 @@ def test_deck_dumps_match_records(tmp_path):
             g = np.load(out / "Dump" / str(k) / "beam_gamma.npy")
             assert g.shape == (NSLICE, 4800)
             assert relerr(g.mean(axis=1), ge[k]) < 1e-12
+
+
+def test_exclusive_harmonics_writes_fundamental_and_highest(tmp_path):
+    """&track exclusive_harmonics = true with bunchharm = 3 writes the
+    bunching of harmonics 1 and 3 only (DiagnosticCUDA.cpp:495-507), with
+    the values of the run without the switch."""
+    print("case deck exclusive_harmonics", flush=True)
+    all_ = tmp_path / "all"
+    excl = tmp_path / "excl"
+    all_.mkdir()
+    excl.mkdir()
+    nrec = api.deck_run(_deck(all_, "bunchharm=3"), all_ / "out")
+    assert api.deck_run(_deck(excl, "bunchharm=3\nexclusive_harmonics=true"), excl / "out") == nrec
+    for name in ("bunching", "bunchingphase"):
+        for h in ("", "3"):
+            a = np.load(excl / "out" / "Beam" / f"{name}{h}.npy")
+            b = np.load(all_ / "out" / "Beam" / f"{name}{h}.npy")
+            assert a.shape == (nrec, NSLICE) and relerr(a, b) < 1e-12, name + h
+        assert (all_ / "out" / "Beam" / f"{name}2.npy").exists()
+        assert not (excl / "out" / "Beam" / f"{name}2.npy").exists()
