@@ -380,6 +380,7 @@ def serving(prec: int, n: int, seed: int, rate_hz):
     st = srv.stats()
     srv.close()
     return {"p50_ms": st["p50_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
+            "p99_over_n": f"p99 of {int(st['completed'])} requests",
             "mean_ms": st["mean_s"] * 1e3, "requests": int(st["completed"]),
             "particle_slice_steps_per_s": st["particle_slice_steps_per_s"],
             "requests_per_s": len(reqs) / wall}
@@ -395,7 +396,7 @@ def main() -> int:
     ap.add_argument("--ngrid", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adi", action="store_true")
     ap.add_argument("--no-precision", action="store_true",

@@ -47,6 +47,11 @@ class G4Error(RuntimeError):
         self.status = status
 
 
+class EField(C.Structure):
+    _fields_ = [("rmax", C.c_double), ("ngrid", C.c_int32), ("nz", C.c_int32),
+                ("nphi", C.c_int32), ("longrange", C.c_int32)]
+
+
 class Step(C.Structure):
     _fields_ = [
         ("delz", C.c_double), ("z", C.c_double), ("aw", C.c_double), ("ku", C.c_double),
@@ -164,6 +169,9 @@ PROTOTYPES = {
     "g4h_beam_cell_index": (C.c_int, [_vp, _vp, _vp, _ip]),
     "g4h_diag_beam_raw": (C.c_int, [_vp, _vp, C.c_int, _dp]),
     "g4h_diag_beam_raw_noalloc": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "g4h_beam_init_efield": (C.c_int, [_vp, C.POINTER(EField), C.c_double]),
+    "g4h_beam_sc_field": (C.c_int, [_vp, _dp]),
+    "g4h_run_download_sc": (C.c_int, [_vp, _dp]),
     "g4h_diag_field_raw": (C.c_int, [_vp, _vp, _dp]),
     "g4h_diag_finalize_beam": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp]),
     "g4h_diag_finalize_field": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _dp,

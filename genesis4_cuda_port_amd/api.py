@@ -193,6 +193,20 @@ class Beam:
         check(lib().g4h_diag_beam_raw(self.ctx.h, self.h, nharm, _dp(out)))
         return out.reshape(self.nslice, beam_raw_len(nharm))
 
+    def init_efield(self, rmax: float, ngrid: int, nz: int, nphi: int, lam: float,
+                    longrange: bool = False):
+        """Short-range space charge on the device before every push
+        (g4h_beam_init_efield); nz = 0 switches it off."""
+        from ._abi import EField
+        ef = EField(rmax, ngrid, nz, nphi, int(longrange))
+        check(lib().g4h_beam_init_efield(self.h, C.byref(ef), lam))
+
+    def sc_field(self) -> np.ndarray:
+        """Mean short-range E_z [V/m] per slice from the last push."""
+        out = np.empty(self.nslice)
+        check(lib().g4h_beam_sc_field(self.h, _dp(out)))
+        return out
+
     def diag_raw_noalloc(self, nharm: int = 1) -> np.ndarray:
         """diag_raw through the entry point that allocates no device memory."""
         out = np.empty(self.nslice * beam_raw_len(nharm))
@@ -322,6 +336,13 @@ class Run:
         n = C.c_int()
         check(lib().g4h_run_records(self.h, C.byref(n)))
         return int(n.value)
+
+    def download_sc(self) -> np.ndarray:
+        """Mean short-range space-charge field per record and slice [V/m]."""
+        n = self.records()
+        out = np.empty(n * self.beam.nslice)
+        check(lib().g4h_run_download_sc(self.h, _dp(out)))
+        return out.reshape(n, self.beam.nslice)
 
     def download_diag(self):
         n = self.records()

@@ -28,6 +28,7 @@ SOURCES = [
     "field_adi.hip",
     "field_misc.hip",
     "loader.hip",
+    "space_charge.hip",
     "abi.cpp",
     "run.cpp",
     "batch.cpp",

@@ -805,8 +805,57 @@ g4h_status g4h_beam_advance(g4h_context* ctx, g4h_beam* b, g4h_field* const* fie
   std::vector<Field*> fl(fields, fields + nfield);
   make_push(p.push, *und, b->gamma_ref, delz, fl.data(), nfield, d_eloss);
   p.push.particle_ez = b->pez_on ? b->pez.get() : nullptr;
+  if (b->ef.nz > 0) {
+    // short-range field from the current positions, the slice loss and a
+    // per-particle term folded in (BeamSolverCUDA.cpp:523-547)
+    const double g0 = b->gamma_ref;
+    launch_space_charge(ctx, b, g0 * g0 / (1.0 + und->aw * und->aw), d_eloss, p.push.particle_ez);
+    p.push.particle_ez = b->scez.get();
+    p.push.slice_eloss = nullptr;
+  }
   launch_particle(ctx, b, p);
   if (slice_eloss) G4_HIP(hipStreamSynchronize(ctx->stream));
+  G4_API_END
+}
+
+g4h_status g4h_beam_init_efield(g4h_beam* b, const g4h_efield* ef, double lambda) {
+  G4_API_BEGIN
+  require(b != nullptr, "beam is null");
+  require(ef != nullptr, "null argument");
+  require(ef->nz >= 0 && ef->nphi >= 0 && ef->ngrid >= 0, "negative solver size");
+  if (ef->nz > 0) {
+    require(ef->ngrid >= 2 && ef->rmax > 0 && lambda > 0,
+            "space charge needs ngrid >= 2, rmax > 0 and lambda > 0");
+    require(!ef->longrange,
+            "the long-range (current-profile) space-charge field is not built; pass it as the "
+            "slice loss");
+    require(sc_lds_bytes(*ef) <= 64 * 1024,
+            "nz * (2 nphi + 1) * ngrid too large for the solver's LDS (at most 2730)");
+  }
+  b->ctx->activate();
+  b->ef = *ef;
+  b->ef_ks = ef->nz > 0 ? 2.0 * M_PI / lambda : 0.0;
+  ++b->ef_generation;
+  if (ef->nz > 0) {
+    if (b->scez.size() != static_cast<size_t>(b->np)) b->scez.alloc(b->ctx, static_cast<size_t>(b->np));
+    if (b->scrms.size() != static_cast<size_t>(b->nslice))
+      b->scrms.alloc(b->ctx, static_cast<size_t>(b->nslice));
+    G4_HIP(hipMemsetAsync(b->scrms.get(), 0, sizeof(double) * b->nslice, b->ctx->stream));
+  }
+  G4_API_END
+}
+
+g4h_status g4h_beam_sc_field(g4h_beam* b, double* field) {
+  G4_API_BEGIN
+  require(b != nullptr, "beam is null");
+  require(field != nullptr, "null argument");
+  b->ctx->activate();
+  if (b->ef.nz <= 0) {
+    for (int s = 0; s < b->nslice; ++s) field[s] = 0.0;
+    return G4H_OK;
+  }
+  G4_HIP(hipStreamSynchronize(b->ctx->stream));
+  G4_HIP(hipMemcpy(field, b->scrms.get(), sizeof(double) * b->nslice, hipMemcpyDeviceToHost));
   G4_API_END
 }
 

@@ -3,9 +3,12 @@
 // time-dependent case, and prints the per-slice power of the first field and
 // the bunching of every output record:
 //
-//   g4adapter_check <fp64|mixed> <adi|fft> <nfield 1|2> <fused 0|1>
+//   g4adapter_check <fp64|mixed> <adi|fft> <nfield 1|2> <fused 0|1> [sc]
 //
-// one line per record: z, then nslice powers, then nslice bunching factors.
+// one line per record: z, then nslice powers, then nslice bunching factors,
+// and with `sc` (short-range space charge through Beam::initEField: rmax 2,
+// 40 shells, 2 harmonics, 3 azimuthal modes) the nslice getSCField values
+// of the operator-by-operator loop.
 // The case: 6 slices x 512 particles (4 mirrored bins, shot noise, seed
 // 4242), gamma0 11350, 3 kA, emittance 0.4 um, beta 15 m, a planar undulator
 // of 12 periods of 26 mm (aw 1.7 rms) with 1.5 m^-2 FODO quadrupoles, a 1e5 W
@@ -20,10 +23,12 @@
 #include "genesis_gpu.hpp"
 
 int main(int argc, char** argv) {
-  if (argc != 5) {
-    std::fprintf(stderr, "usage: %s <fp64|mixed> <adi|fft> <nfield 1|2> <fused 0|1>\n", argv[0]);
+  if (argc != 5 && !(argc == 6 && std::strcmp(argv[5], "sc") == 0)) {
+    std::fprintf(stderr, "usage: %s <fp64|mixed> <adi|fft> <nfield 1|2> <fused 0|1> [sc]\n",
+                 argv[0]);
     return 2;
   }
+  const bool sc = argc == 6;
   const g4h_precision prec = std::strcmp(argv[1], "mixed") == 0 ? G4H_MIXED : G4H_FP64;
   const bool fft = std::strcmp(argv[2], "fft") == 0;
   const int nfield = std::atoi(argv[3]) == 2 ? 2 : 1;
@@ -67,6 +72,7 @@ int main(int argc, char** argv) {
     for (int s = 0; s <= nslice; ++s) off[static_cast<size_t>(s)] = s * npart;
     g4gpu::Beam beam(ctx, prec, off, current, gamma0, lambda0, lambda0);
     beam.pack(x, y, px, py, g, th);
+    if (sc) beam.initEField(2.0, 40, 2, 1, lambda0, false);
     g4gpu::Field f1(ctx, prec, nslice, ngrid, gridmax, lambda0, lambda0, 1);
     f1.pack(re, im, 0);
     f1.initSolver(fft, false, 1.0, 1.0, 1.0);
@@ -86,6 +92,8 @@ int main(int argc, char** argv) {
         std::printf(" %.17g", r.field[0][static_cast<size_t>(s) * G4H_FIELD_Q + G4H_FQ_POWER]);
       for (int s = 0; s < nslice; ++s)
         std::printf(" %.17g", r.beam[static_cast<size_t>(s) * G4H_BEAM_QN(1) + G4H_BEAM_Q]);
+      if (sc && !fused)
+        for (int s = 0; s < nslice; ++s) std::printf(" %.17g", r.sscfield[static_cast<size_t>(s)]);
       std::printf("\n");
     }
   } catch (const std::exception& e) {

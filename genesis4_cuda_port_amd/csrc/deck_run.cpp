@@ -179,6 +179,7 @@ struct Deck {
   int stepsdone = 0;  // integration steps of earlier &track namelists
   int nrecords = 0;
   int ndumps = 0;
+  g4h_efield ef{};  // &efield: short-range space charge
 };
 
 void do_setup(Deck& d, const Namelist& nl) {
@@ -337,6 +338,7 @@ void create_state(Deck& d, RunState& rs, int device) {
   check(g4h_beam_create(rs.ctx, d.prec, d.nslice, d.offsets.data(), d.cur.data(),
                         d.setup.gamma0, &rs.beam));
   check(g4h_beam_upload(rs.beam, x.data(), y.data(), px.data(), py.data(), g.data(), th.data()));
+  if (d.ef.nz > 0) check(g4h_beam_init_efield(rs.beam, &d.ef, d.setup.lambda0));
   std::vector<double> re, im;
   seed_field(d, re, im);
   check(g4h_field_create(rs.ctx, d.prec, d.nslice, d.ngrid, d.gridmax, d.setup.lambda0,
@@ -387,7 +389,8 @@ void write_dump(Deck& d, RunState& rs, const std::string& out, int gstep, double
 
 void write_records(const Deck& d, const std::string& out, const std::vector<double>& zs,
                    const std::vector<double>& braw, const std::vector<double>& fraw, int nrec,
-                   int nharm, bool exclharm, const std::vector<g4h_step>& steps) {
+                   int nharm, bool exclharm, const std::vector<double>& sc,
+                   const std::vector<g4h_step>& steps) {
   const int S = d.nslice;
   const int bq = G4H_BEAM_QN(nharm);
   std::vector<double> q(static_cast<size_t>(nrec) * S * bq), bg(static_cast<size_t>(nrec) * G4H_BEAM_GLOBAL);
@@ -435,6 +438,16 @@ void write_records(const Deck& d, const std::string& out, const std::vector<doub
     write_npy(out + "/Beam/bunching" + suf + ".npy", column(q, bq, G4H_BEAM_Q + 2 * h), nrec, S);
     write_npy(out + "/Beam/bunchingphase" + suf + ".npy", column(q, bq, G4H_BEAM_Q + 2 * h + 1),
               nrec, S);
+  }
+  // longitudinal fields of the auxiliary output (DiagnosticCUDA.cpp:558-570):
+  // wakes (eloss) and the long-range space charge (longESC) are not modelled
+  // by this runner and are zero; sscfield is the device's short-range field
+  {
+    const std::vector<double> zero(static_cast<size_t>(nrec) * S, 0.0);
+    write_npy(out + "/Beam/efield.npy", zero, nrec, S);
+    write_npy(out + "/Beam/wakefield.npy", zero, nrec, S);
+    write_npy(out + "/Beam/lscfield.npy", zero, nrec, S);
+    write_npy(out + "/Beam/sscfield.npy", sc, nrec, S);
   }
   for (int c = 0; c < G4H_FIELD_Q; ++c)
     write_npy(out + "/Field/" + fnames[c] + ".npy", column(fq, G4H_FIELD_Q, c), nrec, S);
@@ -554,6 +567,8 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   std::vector<double> braw(static_cast<size_t>(nrec) * S * G4H_BEAM_RAW(nharm));
   std::vector<double> fraw(static_cast<size_t>(nrec) * S * G4H_FIELD_RAW);
   check(g4h_run_download_diag(run, braw.data(), fraw.data()));
+  std::vector<double> sc(static_cast<size_t>(nrec) * S);
+  check(g4h_run_download_sc(run, sc.data()));
   std::vector<double> zs{d.zdone};
   for (const g4h_step& st : steps)
     if (st.outstep) zs.push_back(st.z);
@@ -561,7 +576,7 @@ void do_track(Deck& d, const Namelist& nl, const std::string& base, const std::s
   d.zdone = steps.back().z;
   d.stepsdone += ns;
   const std::string dir = out + (d.nrecords > 0 ? "/track" + std::to_string(d.nrecords) : "");
-  write_records(d, dir, zs, braw, fraw, nrec, nharm, exclharm, steps);
+  write_records(d, dir, zs, braw, fraw, nrec, nharm, exclharm, sc, steps);
   d.nrecords += nrec;
 }
 
@@ -681,6 +696,18 @@ g4h_status g4h_deck_run(const char* deck_path, const char* out_dir, int device, 
       } else if (nl.name == "beam") {
         if (rs.beam) throw Unsupported("&beam after &track is not supported");
         do_beam(d, nl);
+      } else if (nl.name == "efield") {
+        // &efield (upstream GENESIS 1.3; Beam::initEField, include/Beam.h:126-131)
+        Keys k(nl);
+        d.ef.rmax = k.d("rmax", 0.0);
+        d.ef.ngrid = k.i("ngrid", 100);
+        d.ef.nz = k.i("nz", 0);
+        d.ef.nphi = k.i("nphi", 0);
+        d.ef.longrange = k.b("longrange", false) ? 1 : 0;
+        k.finish();
+        if (d.ef.longrange)
+          throw Unsupported("&efield longrange: the long-range space-charge field is not built");
+        if (rs.beam) check(g4h_beam_init_efield(rs.beam, &d.ef, d.setup.lambda0));
       } else if (nl.name == "track") {
         do_track(d, nl, base, out_dir, rs, device);
       } else {

@@ -287,6 +287,12 @@ struct Beam {
   DevBuf<int4> split_fb;
   DevBuf<int32_t> split_cnt;  // zero between passes
   int max_slice_np = 0;
+  // short-range space charge (g4h_beam_init_efield); ef.nz == 0: off
+  g4h_efield ef{};
+  double ef_ks = 0.0;        // 2 pi / lambda of the solver
+  int ef_generation = 0;     // bumped by every g4h_beam_init_efield
+  DevBuf<double> scez;       // [np] push term of the last space-charge pass
+  DevBuf<double> scrms;      // [nslice] RMS E_z [V/m] over each slice, last pass
 };
 
 struct Field {

@@ -111,6 +111,19 @@ class Beam {
     check(g4h_beam_download(soa_, x.data(), y.data(), px.data(), py.data(), gamma.data(),
                             theta.data()));
   }
+  // Beam::initEField (include/Beam.h:126-131): short-range space charge of
+  // this beam, computed on the device before every push
+  void initEField(double rmax, int ngrid, int nz, int nphi, double lambda, bool longr) {
+    const g4h_efield ef{rmax, ngrid, nz, nphi, longr ? 1 : 0};
+    check(g4h_beam_init_efield(soa_, &ef, lambda));
+  }
+  // Beam::getSCField (include/Beam.h:139-144): RMS short-range E_z [V/m] over
+  // slice islice at the last push
+  double getSCField(int islice) const {
+    std::vector<double> v(static_cast<size_t>(nslice()));
+    check(g4h_beam_sc_field(soa_, v.data()));
+    return v.at(static_cast<size_t>(islice));
+  }
   int nslice() const { return static_cast<int>(offsets.size()) - 1; }
   g4h_beam* soa() const { return soa_; }
   Context& ctx() const { return ctx_; }
@@ -268,9 +281,32 @@ class TrackBeamGPU {
   }
 };
 
-// BeamSolverCUDA (BeamSolverCUDA.cpp:449-683)
+// BeamSolverCUDA (BeamSolverCUDA.cpp:449-687, include/BeamSolverCUDA.h:19-44)
 class BeamSolverGPU {
  public:
+  // BeamSolverCUDA::initEField (include/BeamSolverCUDA.h:42-44): settings of
+  // the short-range space charge, given to the beam at the next advance (the
+  // device beam computes the field itself before its push)
+  void initEField(double rmax, int ngrid, int nz, int nphi, double lambda, bool longr) {
+    ef_ = g4h_efield{rmax, ngrid, nz, nphi, longr ? 1 : 0};
+    ef_lambda_ = lambda;
+    ef_pending_ = true;
+  }
+  // BeamSolverCUDA::checkAllocation (BeamSolverCUDA.cpp:685-687): room for
+  // the short-range field of nslice slices
+  void checkAllocation(unsigned long nslice) { sc_.assign(nslice, 0.0); }
+  // BeamSolverCUDA::getSCField (include/BeamSolverCUDA.h:39): RMS
+  // short-range E_z [V/m] over slice islice at the last advance (0 when
+  // space charge is off or the slice is beyond the allocation)
+  double getSCField(int islice) {
+    if (sc_stale_ && beam_) {
+      sc_.resize(static_cast<size_t>(beam_->nslice()));
+      check(g4h_beam_sc_field(beam_->soa(), sc_.data()));
+      sc_stale_ = false;
+    }
+    return islice >= 0 && static_cast<size_t>(islice) < sc_.size() ? sc_[static_cast<size_t>(islice)]
+                                                                    : 0.0;
+  }
   // slice_eloss: -longESC/511000 per slice (:555); particle_ez: the
   // short-range field per particle plus that loss, beam order (:523-547),
   // which replaces it when given
@@ -279,9 +315,15 @@ class BeamSolverGPU {
     std::vector<g4h_field*> fl;
     for (Field* f : *fields)
       if (f) fl.push_back(f->soa());
+    if (ef_pending_) {
+      check(g4h_beam_init_efield(beam->soa(), &ef_, ef_lambda_));
+      ef_pending_ = false;
+    }
     check(g4h_beam_set_particle_ez(beam->soa(), particle_ez));
     check(g4h_beam_advance(beam->ctx().get(), beam->soa(), fl.data(), static_cast<int>(fl.size()),
                            &und->step(), delz, particle_ez ? nullptr : slice_eloss));
+    beam_ = beam;
+    sc_stale_ = true;
   }
   // the energy change per slice of Collective::apply / Incoherent::apply
   // (Beam.cpp:127-128), applied to the device beam
@@ -293,6 +335,12 @@ class BeamSolverGPU {
 
  private:
   TrackBeamGPU tracker_;
+  g4h_efield ef_{};
+  double ef_lambda_ = 0.0;
+  bool ef_pending_ = false;
+  Beam* beam_ = nullptr;
+  std::vector<double> sc_;
+  bool sc_stale_ = false;
 };
 
 // ControlCUDA (ControlCUDA.cpp:122-252), one rank
@@ -323,6 +371,7 @@ class ControlGPU {
 struct DiagnosticRecord {
   double z = 0;
   std::vector<double> beam, beam_global;                 // nslice x G4H_BEAM_QN, 6
+  std::vector<double> sscfield;                          // nslice (DiagnosticCUDA.cpp:570)
   std::vector<std::vector<double>> field, field_global;  // per field: nslice x 9, 7
 };
 
@@ -338,6 +387,8 @@ class DiagnosticGPU {
     out.beam_global.resize(G4H_BEAM_GLOBAL);
     check(g4h_diag_finalize_beam(S, nharm_, raw.data(), beam->current.data(), beam->gamma_ref,
                                  out.beam.data(), out.beam_global.data()));
+    out.sscfield.resize(static_cast<size_t>(S));
+    check(g4h_beam_sc_field(beam->soa(), out.sscfield.data()));
     out.field.assign(fields->size(), {});
     out.field_global.assign(fields->size(), {});
     for (size_t i = 0; i < fields->size(); ++i) {

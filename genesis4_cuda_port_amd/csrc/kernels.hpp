@@ -151,6 +151,18 @@ void launch_batch_persistent(Context* ctx, g4h_precision prec, int nharm,
 // win_cells on the context's GPU (ngrid <= 512, LDS budget).
 bool batch_persistent_fits(Context* ctx, g4h_precision prec, int nharm, int win_cells, int gmax);
 
+// Short-range space charge of every slice (space_charge.hip): writes
+// b->scez[np] (the push's per-particle longitudinal term: E_z/511000 plus the
+// slice loss d_slice_eloss[s] and d_base[i] when given) and b->scrms[nslice]
+// (RMS E_z [V/m] over each slice's particles).  gz2 = gamma_ref^2/(1+aw^2) of the step.
+struct ScParams {
+  double ks, gz2, rmax, vacimp_over_2pi;
+  int ngrid, nz, nphi, pad_;
+};
+void launch_space_charge(Context* ctx, Beam* b, double gz2, const double* d_slice_eloss,
+                         const double* d_base);
+size_t sc_lds_bytes(const g4h_efield& ef);
+
 // Beam sums into the library's fixed device array (no allocation): blocking,
 // raw[nslice * G4H_BEAM_RAW(nharm)] on the host; fails with
 // G4H_ERR_INVALID_ARGUMENT when the pass needs more than kNoallocScratchBytes.

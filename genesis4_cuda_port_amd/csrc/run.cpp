@@ -43,6 +43,7 @@ struct g4h_run {
   int nrec = 0;
   int cap = 0;
   DevBuf<double> beam_hist, field_hist;
+  DevBuf<double> sc_hist;  // [cap][nslice] mean short-range field per record (zero at record 0)
   // field ring state when the run starts, and the captured run
   int first0 = 0;
   double acc0 = 0.0;
@@ -52,6 +53,7 @@ struct g4h_run {
   hipGraphExec_t exec = nullptr;
   int field_generation = 0;  // of the field when captured
   const double* pez = nullptr;  // per-particle longitudinal field read by the graph
+  int ef_generation = 0;        // of the beam's space-charge settings when captured
   Context::ParamArena arena;
 
   ~g4h_run() {
@@ -99,11 +101,30 @@ struct g4h_run {
       p.diag.raw = beam_hist.get() + nrec * beam_rec();
       p.diag.nharm = cfg.nharm;
     }
+    if (b->ef.nz > 0) {
+      // BeamSolverCUDA::advance takes the short-range field from the
+      // positions after the first transverse half step (Beam.cpp:123-125,
+      // BeamSolverCUDA.cpp:523-547): that half step, the field, then the
+      // rest of the pass with the field as its per-particle term
+      ParticleParams p1{};
+      p1.gamma0 = g0;
+      p1.stages = ST_T1;
+      p1.t1 = p.t1;
+      launch_particle(ctx, b, p1);
+      launch_space_charge(ctx, b, g0 * g0 / (1.0 + st.aw * st.aw), nullptr,
+                          b->pez_on ? b->pez.get() : nullptr);
+      p.stages &= ~ST_T1;
+      p.push.particle_ez = b->scez.get();
+    }
     launch_particle(ctx, b, p);
     field_solve(f, src_on, out);
     slippage_host(f, st.slippage, cfg.sample, cfg.timerun, cfg.periodic);
     if (out) {
       field_map_stats(f, field_hist.get() + nrec * field_rec());
+      if (b->ef.nz > 0 && sc_hist.size() >= static_cast<size_t>(nrec + 1) * b->nslice)
+        G4_HIP(hipMemcpyAsync(sc_hist.get() + static_cast<size_t>(nrec) * b->nslice,
+                              b->scrms.get(), sizeof(double) * b->nslice,
+                              hipMemcpyDeviceToDevice, ctx->stream));
       ++nrec;
     }
   }
@@ -141,6 +162,9 @@ g4h_status g4h_run_create(g4h_context* ctx, g4h_beam* beam, g4h_field* field,
     if (r->cap > 0) {
       r->beam_hist.alloc(ctx, r->cap * r->beam_rec());
       r->field_hist.alloc(ctx, r->cap * r->field_rec());
+      r->sc_hist.alloc(ctx, static_cast<size_t>(r->cap) * beam->nslice);
+      G4_HIP(hipMemsetAsync(r->sc_hist.get(), 0, sizeof(double) * r->cap * beam->nslice,
+                            ctx->stream));
     }
     r->record_initial();
   } catch (...) {
@@ -198,7 +222,8 @@ g4h_status g4h_run_capture(g4h_run* r) {
   G4_HIP(hipStreamSynchronize(ctx->stream));
   ctx->tune_settle();
   if (r->b->prec == G4H_FP64) {  // one parameter block per particle launch
-    const int slots = static_cast<int>(r->steps.size()) + 1;
+    // one parameter block per particle launch: two per step with space charge
+    const int slots = static_cast<int>(r->steps.size()) * (r->b->ef.nz > 0 ? 2 : 1) + 1;
     void* h = nullptr;
     void* d = nullptr;
     G4_HIP(hipHostMalloc(&h, static_cast<size_t>(slots) * Context::kParamBytes, hipHostMallocDefault));
@@ -238,6 +263,7 @@ g4h_status g4h_run_capture(g4h_run* r) {
   G4_HIP(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
   r->field_generation = r->f->generation;
   r->pez = r->b->pez_on ? r->b->pez.get() : nullptr;
+  r->ef_generation = r->b->ef_generation;
   r->done = static_cast<int>(r->steps.size());
   require(r->nrec == r->end_nrec && r->f->first == r->end_first,
           "captured run differs from the eager run");
@@ -255,6 +281,8 @@ g4h_status g4h_run_replay(g4h_run* r) {
           "replaying");
   require((r->b->pez_on ? r->b->pez.get() : nullptr) == r->pez,
           "the beam's per-particle longitudinal field was set or cleared after the capture");
+  require(r->b->ef_generation == r->ef_generation,
+          "the beam's space-charge settings changed after the capture");
   r->ctx->activate();
   r->f->accuslip = r->acc0;
   G4_HIP(hipGraphLaunch(r->exec, r->ctx->stream));
@@ -282,6 +310,17 @@ g4h_status g4h_run_download_diag(g4h_run* r, double* beam_raw, double* field_raw
                      hipMemcpyDeviceToHost));
   if (r->nrec > 0 && field_raw)
     G4_HIP(hipMemcpy(field_raw, r->field_hist.get(), sizeof(double) * r->nrec * r->field_rec(),
+                     hipMemcpyDeviceToHost));
+  G4_API_END
+}
+
+g4h_status g4h_run_download_sc(g4h_run* r, double* sc_field) {
+  G4_API_BEGIN
+  require(r && sc_field, "null argument");
+  r->ctx->activate();
+  G4_HIP(hipStreamSynchronize(r->ctx->stream));
+  if (r->nrec > 0)
+    G4_HIP(hipMemcpy(sc_field, r->sc_hist.get(), sizeof(double) * r->nrec * r->b->nslice,
                      hipMemcpyDeviceToHost));
   G4_API_END
 }

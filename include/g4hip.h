@@ -196,6 +196,27 @@ g4h_status g4h_beam_set_particle_ez(g4h_beam* beam, const double* ez);
  * device state instead.  Enqueued on the context stream. */
 g4h_status g4h_beam_add_energy(g4h_context* ctx, g4h_beam* beam, const double* dgamma);
 
+/* BeamSolverCUDA::initEField (include/BeamSolverCUDA.h:42-44), called by
+ * Beam::initEField (include/Beam.h:126-131) from the deck's &efield: the
+ * short-range space charge of the beam (g4h_efield, g4hip_types.h).  The
+ * reference computes it on the host from the host particle arrays
+ * (EFieldSolver::shortRange, BeamSolverCUDA.cpp:523-547); here it is computed
+ * on the device from the device beam before every push, by g4h_beam_advance
+ * (from the positions it is given) and by g4h_run_advance (after the first
+ * transverse half step, where BeamSolverCUDA::advance computes it), and added
+ * to the slice loss and to a per-particle field set with
+ * g4h_beam_set_particle_ez.  lambda is the radiation wavelength (ks = 2
+ * pi/lambda).  nz = 0 switches it off; longrange != 0 is rejected (the
+ * long-range field enters as the slice loss). */
+g4h_status g4h_beam_init_efield(g4h_beam* beam, const g4h_efield* efield, double lambda);
+
+/* BeamSolverCUDA::getSCField (include/BeamSolverCUDA.h:39, read for the
+ * sscfield output at DiagnosticCUDA.cpp:570): per slice, the RMS short-range
+ * E_z [V/m] over its particles from the last push (its mean vanishes to first
+ * order; zeros when space charge is off).  field[nslice] on the host;
+ * blocking. */
+g4h_status g4h_beam_sc_field(g4h_beam* beam, double* field);
+
 /* Field::track = solver->init + solver->advance (src/Core/Field.cpp:204-212;
  * FieldSolverADICUDA::advance :638-818, FieldSolverFFTCUDA::advance
  * :325-483): source deposition from the beam, then one ADI or FFT step. */
@@ -283,6 +304,11 @@ g4h_status g4h_run_capture(g4h_run* run);
 g4h_status g4h_run_replay(g4h_run* run);
 /* Number of diagnostic records written so far. */
 g4h_status g4h_run_records(const g4h_run* run, int* nrecords);
+/* Per record and slice, the RMS short-range space-charge field [V/m] of the
+ * record's step (the sscfield output, DiagnosticCUDA.cpp:570, from
+ * Beam::getSCField); zeros for record 0 and when space charge is off.
+ * sc_field[nrecords * nslice]; blocking. */
+g4h_status g4h_run_download_sc(g4h_run* run, double* sc_field);
 /* Blocking copy of records [0, nrecords) (layouts as g4h_diag_*_raw). */
 g4h_status g4h_run_download_diag(g4h_run* run, double* beam_raw, double* field_raw);
 

@@ -28,6 +28,17 @@ extern "C" {
 #define G4H_CE 4.8032e-11            /* e*c [A m] */
 #define G4H_CLIGHT 299792458.0       /* speed of light [m/s] */
 
+/* Short-range space-charge solver (BeamSolverCUDA::initEField,
+ * include/BeamSolverCUDA.h:42-44; the deck's &efield namelist): per slice,
+ * nz longitudinal harmonics of the phase (0: off) times 2*nphi+1 azimuthal
+ * modes around the slice centroid, on ngrid radial shells out to rmax times
+ * the slice's largest radial offset. */
+typedef struct g4h_efield {
+  double rmax;
+  int32_t ngrid, nz, nphi;
+  int32_t longrange; /* the long-range (current-profile) field: not built */
+} g4h_efield;
+
 /* Status codes: every C-ABI call returns one; no exception crosses the ABI. */
 typedef enum g4h_status {
   G4H_OK = 0,

@@ -18,6 +18,7 @@
 
 #include "g4oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -988,16 +989,134 @@ void orc_finalize_field(int nslice, int ngrid, double dgrid, double xlambda,
 }
 
 // Gencore::run loop body (Gencore.cpp:172-254) for a single field.
-int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
-            double* py, double* gamma, double* theta, double gamma0, double lambda0,
-            const double* current, const double* slice_eloss, orc_field* field,
-            int solver, int do_filter, double xc, double yc, double sig,
-            const g4h_step* steps, int nsteps, double sample, int timerun,
-            int periodic, int nharm, double* beam_raw, double* field_raw) {
+// Short-range space charge (EFieldSolver::shortRange, called by the
+// reference at BeamSolverCUDA.cpp:523-547; upstream GENESIS 1.3 owns it and
+// it is absent from the reference tree), restated from the published method.
+// Per slice, around the centroid (xc, yc) of its particles: radial shells
+// n = 0..N-1 of width dr = rmax * max|r - rc| / N; the charge density of the
+// slice (charge I*slicelength/c in macro particles of equal charge, the phase
+// structure periodic in the radiation wavelength) is expanded in harmonics
+// l = 1..nz of the phase theta and azimuthal modes m = -nphi..nphi:
+//   S_lmn = sum over particles in shell n of exp(-i (l theta + m phi)).
+// The mode's potential solves (1/r)(r u')' - (m^2/r^2 + kappa^2) u =
+// -rho/eps0 with kappa = l ks / gz (the Lorentz factor of the longitudinal
+// motion, gz2 = gamma0^2/(1+aw^2)), u = 0 at r = N dr, discretised at the
+// shell centres r_n = (n+1/2) dr with fluxes through the shell faces n dr and
+// (n+1) dr:
+//   n u_{n-1} - [2n+1 + m^2/(n+1/2) + (n+1/2)(kappa dr)^2] u_n + (n+1) u_{n+1}
+//     = (I vacimp / (2 pi Np)) S_lmn                      (charge -e)
+// (Thomas), and the field of harmonic l in a relativistic beam is
+// E_z = -i (l ks / gz2) u e^{i(l theta + m phi)} + c.c.  The push takes
+// E_z/511000 (BeamSolverCUDA.cpp:532) as its longitudinal term.
+void orc_sc_field(int nslice, const int32_t* off, const double* x, const double* y,
+                  const double* theta, const double* current, double ks, double gz2,
+                  const g4h_efield* ef, double* ez, double* slice_rms) {
+  const int N = ef ? ef->ngrid : 0;
+  const int nz = ef ? ef->nz : 0;
+  const int nphi = ef ? ef->nphi : 0;
+  const int nm = 2 * nphi + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int is = 0; is < nslice; ++is) {
+    const int64_t i0 = off[is], i1 = off[is + 1];
+    const int64_t np = i1 - i0;
+    for (int64_t i = i0; i < i1; ++i) ez[i] = 0.0;
+    if (slice_rms) slice_rms[is] = 0.0;
+    if (np <= 0 || nz <= 0 || N <= 0 || current[is] == 0.0) continue;
+    double sx = 0.0, sy = 0.0;
+    for (int64_t i = i0; i < i1; ++i) {
+      sx += x[i];
+      sy += y[i];
+    }
+    const double xc = sx / static_cast<double>(np), yc = sy / static_cast<double>(np);
+    double r2 = 0.0;
+    for (int64_t i = i0; i < i1; ++i) {
+      const double dx = x[i] - xc, dy = y[i] - yc;
+      r2 = std::max(r2, dx * dx + dy * dy);
+    }
+    const double dr = ef->rmax * std::sqrt(r2) / N;
+    if (!(dr > 0.0)) continue;
+    std::vector<int> shell(static_cast<size_t>(np));
+    std::vector<std::complex<double>> eth(static_cast<size_t>(np)), eph(static_cast<size_t>(np));
+    std::vector<std::complex<double>> S(static_cast<size_t>(nz) * nm * N, 0.0);
+    for (int64_t i = i0; i < i1; ++i) {
+      const double dx = x[i] - xc, dy = y[i] - yc;
+      const double r = std::sqrt(dx * dx + dy * dy);
+      const size_t k = static_cast<size_t>(i - i0);
+      shell[k] = std::min(static_cast<int>(r / dr), N - 1);
+      eph[k] = r > 0.0 ? std::complex<double>(dx / r, dy / r) : std::complex<double>(1.0, 0.0);
+      eth[k] = std::complex<double>(std::cos(theta[i]), std::sin(theta[i]));
+      std::complex<double> el = 1.0;
+      for (int l = 1; l <= nz; ++l) {
+        el *= eth[k];
+        // em = e^{i m phi} for m = -nphi..nphi; the source term is its
+        // product with e^{i l theta}, conjugated
+        std::complex<double> em = 1.0;
+        for (int q = 0; q < nphi; ++q) em *= std::conj(eph[k]);  // e^{-i nphi phi}
+        for (int m = -nphi; m <= nphi; ++m) {
+          S[(static_cast<size_t>(l - 1) * nm + (m + nphi)) * N + shell[k]] += std::conj(el * em);
+          em *= eph[k];
+        }
+      }
+    }
+    const double d0 = current[is] * kVacimp / (2.0 * M_PI * static_cast<double>(np));
+    std::vector<double> cp(static_cast<size_t>(N));
+    std::vector<std::complex<double>> u(static_cast<size_t>(N));
+    for (int l = 1; l <= nz; ++l) {
+      const double kd = l * ks * dr;
+      const double kd2 = kd * kd / gz2;
+      for (int m = -nphi; m <= nphi; ++m) {
+        std::complex<double>* s = &S[(static_cast<size_t>(l - 1) * nm + (m + nphi)) * N];
+        // Thomas: a_n = n, b_n, c_n = n + 1, u_N = 0
+        for (int n = 0; n < N; ++n) {
+          const double h = n + 0.5;
+          const double b = -(2.0 * n + 1.0) - static_cast<double>(m * m) / h - h * kd2;
+          const double a = static_cast<double>(n);
+          const double den = n == 0 ? b : b - a * cp[static_cast<size_t>(n - 1)];
+          cp[static_cast<size_t>(n)] = (n + 1.0) / den;
+          const std::complex<double> d = d0 * s[n];
+          u[static_cast<size_t>(n)] = n == 0 ? d / den : (d - a * u[static_cast<size_t>(n - 1)]) / den;
+        }
+        for (int n = N - 2; n >= 0; --n)
+          u[static_cast<size_t>(n)] -= cp[static_cast<size_t>(n)] * u[static_cast<size_t>(n + 1)];
+        for (int n = 0; n < N; ++n) s[n] = u[static_cast<size_t>(n)];
+      }
+    }
+    double esum = 0.0;
+    for (int64_t i = i0; i < i1; ++i) {
+      const size_t k = static_cast<size_t>(i - i0);
+      double e = 0.0;
+      std::complex<double> el = 1.0;
+      for (int l = 1; l <= nz; ++l) {
+        el *= eth[k];
+        std::complex<double> em = 1.0;
+        for (int q = 0; q < nphi; ++q) em *= std::conj(eph[k]);
+        for (int m = -nphi; m <= nphi; ++m) {
+          const std::complex<double> uu =
+              S[(static_cast<size_t>(l - 1) * nm + (m + nphi)) * N + shell[k]];
+          // 2 Re[-i (l ks/gz2) u e^{i(l theta + m phi)}] = 2 (l ks/gz2) Im[u e^{..}]
+          e += 2.0 * (l * ks / gz2) * std::imag(uu * el * em);
+          em *= eph[k];
+        }
+      }
+      ez[i] = e / 511000.0;
+      esum += e * e;
+    }
+    if (slice_rms) slice_rms[is] = std::sqrt(esum / static_cast<double>(np));
+  }
+}
+
+static int run_impl(int nslice, const int32_t* off, double* x, double* y, double* px,
+                    double* py, double* gamma, double* theta, double gamma0, double lambda0,
+                    const double* current, const double* slice_eloss, orc_field* field,
+                    int solver, int do_filter, double xc, double yc, double sig,
+                    const g4h_step* steps, int nsteps, double sample, int timerun,
+                    int periodic, int nharm, double* beam_raw, double* field_raw,
+                    const g4h_efield* ef) {
   const int64_t np = off[nslice];
   const int64_t plane = static_cast<int64_t>(field->ngrid) * field->ngrid;
   const int rs = G4H_BEAM_RAW(nharm);
   std::vector<double> sre(plane * field->nslice), sim(plane * field->nslice);
+  std::vector<double> ez(ef && ef->nz > 0 ? static_cast<size_t>(np) : 0);
   int rec = 0;
   auto record = [&]() {
     if (beam_raw)
@@ -1013,8 +1132,19 @@ int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
     const double delz = st->delz;
     // Beam::track (Beam.cpp:111-134)
     orc_track_transverse(np, x, y, px, py, gamma, st, gamma0, delz * 0.5, 0);
-    orc_beam_advance(nslice, off, x, y, px, py, gamma, theta, field, 1, st, gamma0, delz,
-                     slice_eloss);
+    if (ef && ef->nz > 0) {
+      // BeamSolverCUDA::advance: short-range field + slice loss per particle
+      const double gz2 = gamma0 * gamma0 / (1.0 + st->aw * st->aw);
+      orc_sc_field(nslice, off, x, y, theta, current, field->xks, gz2, ef, ez.data(), nullptr);
+      if (slice_eloss)
+        for (int is = 0; is < nslice; ++is)
+          for (int64_t ip = off[is]; ip < off[is + 1]; ++ip) ez[ip] += slice_eloss[is];
+      orc_beam_advance_ez(nslice, off, x, y, px, py, gamma, theta, field, 1, st, gamma0, delz,
+                          ez.data());
+    } else {
+      orc_beam_advance(nslice, off, x, y, px, py, gamma, theta, field, 1, st, gamma0, delz,
+                       slice_eloss);
+    }
     orc_apply_r56(np, gamma, theta, st, gamma0, lambda0);
     orc_track_transverse(np, x, y, px, py, gamma, st, gamma0, delz * 0.5, 1);
     // Field::track (Field.cpp:204-212)
@@ -1030,6 +1160,29 @@ int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
     if (st->outstep) record();
   }
   return rec;
+}
+
+int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
+            double* py, double* gamma, double* theta, double gamma0, double lambda0,
+            const double* current, const double* slice_eloss, orc_field* field,
+            int solver, int do_filter, double xc, double yc, double sig,
+            const g4h_step* steps, int nsteps, double sample, int timerun,
+            int periodic, int nharm, double* beam_raw, double* field_raw) {
+  return run_impl(nslice, off, x, y, px, py, gamma, theta, gamma0, lambda0, current, slice_eloss,
+                  field, solver, do_filter, xc, yc, sig, steps, nsteps, sample, timerun, periodic,
+                  nharm, beam_raw, field_raw, nullptr);
+}
+
+int orc_run_sc(int nslice, const int32_t* off, double* x, double* y, double* px,
+               double* py, double* gamma, double* theta, double gamma0, double lambda0,
+               const double* current, const double* slice_eloss, orc_field* field,
+               int solver, int do_filter, double xc, double yc, double sig,
+               const g4h_step* steps, int nsteps, double sample, int timerun,
+               int periodic, int nharm, double* beam_raw, double* field_raw,
+               const g4h_efield* ef) {
+  return run_impl(nslice, off, x, y, px, py, gamma, theta, gamma0, lambda0, current, slice_eloss,
+                  field, solver, do_filter, xc, yc, sig, steps, nsteps, sample, timerun, periodic,
+                  nharm, beam_raw, field_raw, ef);
 }
 
 }  // extern "C"

@@ -38,6 +38,11 @@
  *   orc_diag_*, orc_finalize_*  quiet-start moments and zero bunching;
  *                          shot-noise statistics; Gaussian-mode power, size
  *   orc_run                beam-to-field energy balance (first order in delz)
+ *   orc_sc_field           uniform modulated disk: potential and field equal
+ *                          the Bessel-function solution, second order in the
+ *                          shell width; leading particles of a bunch gain
+ *                          energy; linear in current; zero for a quiet start;
+ *                          azimuthal modes vanish for a symmetric beam
  * Parity unpinned: the physical constants (vacimp, eev, ce) and fc(h) follow upstream
  * definitions that are absent from the reference.
  */
@@ -151,6 +156,17 @@ void orc_finalize_field(int nslice, int ngrid, double dgrid, double xlambda,
                         double slicelength, const double* raw, double* q,
                         double* global);
 
+/* Short-range longitudinal space-charge field of every particle, the
+ * EFieldSolver::shortRange the reference runs on the host before each push
+ * (BeamSolverCUDA.cpp:523-547; upstream GENESIS 1.3, not in the reference),
+ * restated from its published method (see the .cpp).  ks: radiation
+ * wavenumber of the fundamental; gz2 = gamma0^2/(1+aw^2) (:527).  ez[np]
+ * receives E_z/511000 (the push's units, :532), slice_rms[nslice] (may be
+ * NULL) the RMS E_z [V/m] over each slice's particles. */
+void orc_sc_field(int nslice, const int32_t* off, const double* x, const double* y,
+                  const double* theta, const double* current, double ks, double gz2,
+                  const g4h_efield* ef, double* ez, double* slice_rms);
+
 /* The Gencore::run step loop (Gencore.cpp:161-254, Beam::track Beam.cpp:
  * 111-134, Field::track Field.cpp:204-212) for one field.  Records raw
  * diagnostics before the loop and after every outstep.  beam_raw capacity:
@@ -162,6 +178,18 @@ int orc_run(int nslice, const int32_t* off, double* x, double* y, double* px,
             int solver, int do_filter, double xc, double yc, double sig,
             const g4h_step* steps, int nsteps, double sample, int timerun,
             int periodic, int nharm, double* beam_raw, double* field_raw);
+
+/* orc_run with the short-range space-charge field of orc_sc_field computed
+ * after the first transverse half step of every step (where
+ * BeamSolverCUDA::advance computes it) and added to slice_eloss in the push;
+ * ef may be NULL (= orc_run). */
+int orc_run_sc(int nslice, const int32_t* off, double* x, double* y, double* px,
+               double* py, double* gamma, double* theta, double gamma0, double lambda0,
+               const double* current, const double* slice_eloss, orc_field* field,
+               int solver, int do_filter, double xc, double yc, double sig,
+               const g4h_step* steps, int nsteps, double sample, int timerun,
+               int periodic, int nharm, double* beam_raw, double* field_raw,
+               const g4h_efield* ef);
 
 #ifdef __cplusplus
 }

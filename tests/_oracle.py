@@ -10,7 +10,7 @@ from pathlib import Path
 
 import numpy as np
 
-from genesis4_cuda_port_amd._abi import Step, beam_raw_len, FIELD_RAW
+from genesis4_cuda_port_amd._abi import EField, Step, beam_raw_len, FIELD_RAW
 
 ORACLE_LIB = Path(__file__).resolve().parent.parent / "oracle" / "_build" / "libg4oracle.so"
 
@@ -59,6 +59,12 @@ PROTO = {
     "orc_run": (C.c_int, [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                           _dp, _dp, _fp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                           _sp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "orc_run_sc": (C.c_int, [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                             _dp, _dp, _fp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                             _sp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                             C.POINTER(EField)]),
+    "orc_sc_field": (None, [C.c_int, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                            C.POINTER(EField), _dp, _dp]),
 }
 
 _LIB = None
@@ -108,17 +114,35 @@ class OField:
         return self.s.first
 
 
+def sc_field(beam, ks, gz2, rmax, ngrid, nz, nphi):
+    """orc_sc_field: (per-particle push term E_z/511000, per-slice RMS E_z)."""
+    S = beam.offsets.size - 1
+    ez = np.empty(beam.x.size)
+    rms = np.empty(S)
+    ef = EField(rmax, ngrid, nz, nphi, 0)
+    lib().orc_sc_field(S, ip(beam.offsets), dp(beam.x), dp(beam.y), dp(beam.theta),
+                       dp(np.ascontiguousarray(beam.current, dtype=np.float64)), ks, gz2,
+                       C.byref(ef), dp(ez), dp(rms))
+    return ez, rms
+
+
 def run(beam, field: OField, steps, *, gamma0, lambda0, solver, sample=1.0, timerun=True,
-        periodic=False, nharm=1, do_filter=False, xc=1.0, yc=1.0, sig=1.0, slice_eloss=None):
-    """orc_run on copies of the beam arrays; returns (beam arrays, braw, fraw)."""
+        periodic=False, nharm=1, do_filter=False, xc=1.0, yc=1.0, sig=1.0, slice_eloss=None,
+        efield=None):
+    """orc_run on copies of the beam arrays; returns (beam arrays, braw, fraw).
+    efield = (rmax, ngrid, nz, nphi) adds the short-range space charge."""
     S = beam.offsets.size - 1
     b = beam.copy()
     nout = 1 + sum(1 for s in steps if s.outstep)
     braw = np.zeros(nout * S * beam_raw_len(nharm))
     fraw = np.zeros(nout * S * FIELD_RAW)
     el = None if slice_eloss is None else dp(np.ascontiguousarray(slice_eloss, dtype=np.float64))
-    n = lib().orc_run(S, ip(b.offsets), dp(b.x), dp(b.y), dp(b.px), dp(b.py), dp(b.gamma),
-                      dp(b.theta), gamma0, lambda0, dp(b.current), el, field.ptr, solver,
-                      int(do_filter), xc, yc, sig, steps, len(steps), sample, int(timerun),
-                      int(periodic), nharm, dp(braw), dp(fraw))
+    args = (S, ip(b.offsets), dp(b.x), dp(b.y), dp(b.px), dp(b.py), dp(b.gamma), dp(b.theta),
+            gamma0, lambda0, dp(b.current), el, field.ptr, solver, int(do_filter), xc, yc, sig,
+            steps, len(steps), sample, int(timerun), int(periodic), nharm, dp(braw), dp(fraw))
+    if efield is None:
+        n = lib().orc_run(*args)
+    else:
+        ef = EField(float(efield[0]), int(efield[1]), int(efield[2]), int(efield[3]), 0)
+        n = lib().orc_run_sc(*args, C.byref(ef))
     return b, braw.reshape(n, S, -1), fraw.reshape(n, S, FIELD_RAW)

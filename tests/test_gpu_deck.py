@@ -143,3 +143,41 @@ def test_exclusive_harmonics_writes_fundamental_and_highest(tmp_path):
             assert a.shape == (nrec, NSLICE) and relerr(a, b) < 1e-12, name + h
         assert (all_ / "out" / "Beam" / f"{name}2.npy").exists()
         assert not (excl / "out" / "Beam" / f"{name}2.npy").exists()
+
+
+def test_efield_namelist_runs_device_space_charge(tmp_path):
+    """&efield (rmax, ngrid, nz, nphi) before &track: the deck's run equals the
+    oracle's run with the same short-range space charge, and the records hold
+    sscfield (non-zero after the first step) beside zero efield, wakefield and
+    lscfield (no wakes or long-range field in this runner)."""
+    print("case deck efield", flush=True)
+    deck = _deck(tmp_path, "")
+    text = deck.read_text().replace("&track", "&efield\nrmax=2.0\nngrid=40\nnz=2\nnphi=1\n&end\n\n"
+                                               "&track")
+    deck.write_text(text)
+    out = tmp_path / "out"
+    nrec = api.deck_run(deck, out)
+    meta = json.loads((out / "meta.json").read_text())
+    steps = api.lattice_parse((EX / "fel.lat").read_text(), "FEL", 0.24, LAMBDA0, GAMMA0, 1, ZSTOP)
+    beam = api.load_beam(nslice=NSLICE, npart=4800, nbins=4, gamma0=GAMMA0,
+                         espread_rel=2.24 / GAMMA0, emitn=1.2e-6, beta=meta["betax"],
+                         alpha=meta["alphax"], beta_y=meta["betay"], alpha_y=meta["alphay"],
+                         slicelength=SAMPLE * LAMBDA0, current=3400.0, shotnoise=True,
+                         seed=1234567)
+    of = OField(NSLICE, 127, 120e-6, LAMBDA0, SAMPLE * LAMBDA0)
+    _, braw, fraw = _oracle.run(beam, of, steps, gamma0=GAMMA0, lambda0=LAMBDA0, solver=api.ADI,
+                                sample=SAMPLE, timerun=True, nharm=1, efield=(2.0, 40, 2, 1))
+    dgrid = 2 * 120e-6 / 126
+    P = np.array([api.finalize_field(fraw[r], dgrid, LAMBDA0, SAMPLE * LAMBDA0)[0][:, 0]
+                  for r in range(nrec)])
+    B = np.array([api.finalize_beam(braw[r], 1, np.full(NSLICE, 3400.0), GAMMA0)[0][:, 25]
+                  for r in range(nrec)])
+    gp = np.load(out / "Field" / "power.npy")
+    gb = np.load(out / "Beam" / "bunching.npy")
+    ep, eb = relerr(gp[1:], P[1:]), relerr(gb, B)
+    print(f"  power rel err {ep:.3e}  bunching rel err {eb:.3e}", flush=True)
+    assert ep < 1e-10 and eb < 1e-10
+    ssc = np.load(out / "Beam" / "sscfield.npy")
+    assert ssc.shape == (nrec, NSLICE) and np.all(ssc[0] == 0) and np.any(ssc[1:] != 0)
+    for name in ("efield", "wakefield", "lscfield"):
+        assert np.array_equal(np.load(out / "Beam" / f"{name}.npy"), np.zeros((nrec, NSLICE)))
