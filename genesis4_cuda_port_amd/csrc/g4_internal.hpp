@@ -114,11 +114,11 @@ enum TuneFamily : int {
 };
 
 // Autotuner: for each (family, shape) the variants are tried in turn on live
-// launches (kReps times each), timed with events that are read only once
-// they have completed (no host stall), and the fastest is kept for the rest
-// of the context's life.  A family can be pinned to one variant.
+// launches (kReps times each), timed with events that the host waits for
+// once, after the last trial, and the fastest is kept for the rest of the
+// context's life.  A family can be pinned to one variant.
 struct Tuner {
-  static constexpr int kReps = 2;
+  static constexpr int kReps = 3;
   struct Trial {
     int variant;
     hipEvent_t a, b;

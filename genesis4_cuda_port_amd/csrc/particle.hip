@@ -175,6 +175,18 @@ __device__ __forceinline__ void plane_step(int mode, float str, float off, float
   mom = a3 * xt * gz + a1 * mom;
 }
 
+// An FP64 constant in a scalar register pair, materialized where it is used.
+// gfx950 has no 64-bit literal operands; left to itself the compiler builds
+// each polynomial coefficient in a vector register pair (two v_mov_b32) and
+// accumulates with v_fmac_f64: three vector instructions per Horner step.
+// Passed through a scalar move that cannot be hoisted, the coefficient is the
+// scalar operand of one v_fma_f64, and its moves run on the scalar unit.
+__device__ __forceinline__ double sconst(double c) {
+  double r;
+  asm volatile("s_mov_b64 %0, %1" : "=s"(r) : "s"(c));
+  return r;
+}
+
 // FP64 with `poly`: the power series of the MIXED overload below taken to
 // w^8 (for |w| <= 1/2 the remainders are below 1e-18, under FP64
 // rounding), with one reciprocal of gammaz for both planes; otherwise the
@@ -192,23 +204,23 @@ __device__ __forceinline__ void plane_step(int mode, double str, double off, dou
   }
   const double k = str * igz;
   const double w = k * delz * delz;
-  double c = 1.0 / 20922789888000.0;  // cos(sqrt(w)): (-w)^n / (2n)!, n <= 8
-  c = fma(c, w, -1.0 / 87178291200.0);
-  c = fma(c, w, 1.0 / 479001600.0);
-  c = fma(c, w, -1.0 / 3628800.0);
-  c = fma(c, w, 1.0 / 40320.0);
-  c = fma(c, w, -1.0 / 720.0);
-  c = fma(c, w, 1.0 / 24.0);
+  double c = sconst(1.0 / 20922789888000.0);  // cos(sqrt(w)): (-w)^n / (2n)!, n <= 8
+  c = fma(c, w, sconst(-1.0 / 87178291200.0));
+  c = fma(c, w, sconst(1.0 / 479001600.0));
+  c = fma(c, w, sconst(-1.0 / 3628800.0));
+  c = fma(c, w, sconst(1.0 / 40320.0));
+  c = fma(c, w, sconst(-1.0 / 720.0));
+  c = fma(c, w, sconst(1.0 / 24.0));
   c = fma(c, w, -0.5);
   c = fma(c, w, 1.0);
-  double sn = 1.0 / 355687428096000.0;  // sin(sqrt(w))/sqrt(w): (-w)^n / (2n+1)!
-  sn = fma(sn, w, -1.0 / 1307674368000.0);
-  sn = fma(sn, w, 1.0 / 6227020800.0);
-  sn = fma(sn, w, -1.0 / 39916800.0);
-  sn = fma(sn, w, 1.0 / 362880.0);
-  sn = fma(sn, w, -1.0 / 5040.0);
-  sn = fma(sn, w, 1.0 / 120.0);
-  sn = fma(sn, w, -1.0 / 6.0);
+  double sn = sconst(1.0 / 355687428096000.0);  // sin(sqrt(w))/sqrt(w): (-w)^n / (2n+1)!
+  sn = fma(sn, w, sconst(-1.0 / 1307674368000.0));
+  sn = fma(sn, w, sconst(1.0 / 6227020800.0));
+  sn = fma(sn, w, sconst(-1.0 / 39916800.0));
+  sn = fma(sn, w, sconst(1.0 / 362880.0));
+  sn = fma(sn, w, sconst(-1.0 / 5040.0));
+  sn = fma(sn, w, sconst(1.0 / 120.0));
+  sn = fma(sn, w, sconst(-1.0 / 6.0));
   sn = fma(sn, w, 1.0);
   const double a2 = delz * sn;  // sin(omega)/foc
   const double xt = coord - off;
@@ -398,19 +410,19 @@ __device__ __forceinline__ void sincos_near(double d, double a, double s0, doubl
     return;
   }
   const double d2 = d * d;
-  double ps = -1.0 / 6227020800.0;  // -1/13!
-  ps = fma(ps, d2, 1.0 / 39916800.0);
-  ps = fma(ps, d2, -1.0 / 362880.0);
-  ps = fma(ps, d2, 1.0 / 5040.0);
-  ps = fma(ps, d2, -1.0 / 120.0);
-  ps = fma(ps, d2, 1.0 / 6.0);
+  double ps = sconst(-1.0 / 6227020800.0);  // -1/13!
+  ps = fma(ps, d2, sconst(1.0 / 39916800.0));
+  ps = fma(ps, d2, sconst(-1.0 / 362880.0));
+  ps = fma(ps, d2, sconst(1.0 / 5040.0));
+  ps = fma(ps, d2, sconst(-1.0 / 120.0));
+  ps = fma(ps, d2, sconst(1.0 / 6.0));
   const double sd = fma(-ps * d2, d, d);  // d - d^3/3! + ...
-  double pc = 1.0 / 87178291200.0;        // 1/14!
-  pc = fma(pc, d2, -1.0 / 479001600.0);
-  pc = fma(pc, d2, 1.0 / 3628800.0);
-  pc = fma(pc, d2, -1.0 / 40320.0);
-  pc = fma(pc, d2, 1.0 / 720.0);
-  pc = fma(pc, d2, -1.0 / 24.0);
+  double pc = sconst(1.0 / 87178291200.0);  // 1/14!
+  pc = fma(pc, d2, sconst(-1.0 / 479001600.0));
+  pc = fma(pc, d2, sconst(1.0 / 3628800.0));
+  pc = fma(pc, d2, sconst(-1.0 / 40320.0));
+  pc = fma(pc, d2, sconst(1.0 / 720.0));
+  pc = fma(pc, d2, sconst(-1.0 / 24.0));
   pc = fma(pc, d2, 0.5);
   const double cd = fma(-pc, d2, 1.0);  // 1 - d^2/2! + ...
   s = fma(s0, cd, c0 * sd);
@@ -422,26 +434,54 @@ __device__ __forceinline__ void sincos_near(double d, double a, double s0, doubl
 // factors exp(-i h theta) come from the step's initial ones (bs, bc at
 // theta0 = tthet - dth) through sincos_near, and 1/beta_par through
 // inv_btpar, rounded as the reference rounds it.
-template <int NF>
+template <bool ACC, int NF>
 __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, double tthet,
                                       double dth, const double* bs, const double* bc, double btpar,
                                       double ez, const double* rr, const double* ri,
                                       double& k2gg, double& k2pp) {
   const double ztemp1 = -2.0 / p.xks;
-  double cre = 0.0, cim = 0.0;
+  // the field sums start from the first field's term, not from 0 + term
+  // (an FP64 add the compiler may not drop: 0 + -0 is +0)
+  double cre, cim;
 #pragma unroll
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
     double s, c;
     sincos_near(p.f[i].rharm * dth, p.f[i].rharm * tthet, bs[i], bc[i], s, c);
-    cre += rr[i] * c + ri[i] * s;
-    cim += ri[i] * c - rr[i] * s;
+    if (i == 0) {
+      cre = rr[i] * c + ri[i] * s;
+      cim = ri[i] * c - rr[i] * s;
+    } else {
+      cre += rr[i] * c + ri[i] * s;
+      cim += ri[i] * c - rr[i] * s;
+    }
   }
   const double btper0 = btpar + ztemp1 * cre;
   const double itg = rcp_nr(tgam);
   const double ib = inv_btpar(btper0, tgam * tgam, itg * itg);  // 1/btpar0
-  k2pp += p.xks * (1.0 - ib) + p.xku;
-  k2gg += cim * ib * itg - ez;
+  // ACC: add to the stage sums; else they are zero before the call and take
+  // the values themselves
+  const double dpp = p.xks * (1.0 - ib) + p.xku;
+  const double dgg = cim * ib * itg - ez;
+  if (ACC) {
+    k2pp += dpp;
+    k2gg += dgg;
+  } else {
+    k2pp = dpp;
+    k2gg = dgg;
+  }
+}
+
+// fl(x / 6) for the RK4 weights, correctly rounded as the division is:
+// q0 = x * fl(1/3) is within an ulp of x/3, the residual x - 3 q0 is exact in
+// an FMA, and q0 + r * fl(1/3) rounds to fl(x / 3) (Markstein's theorem: a
+// correctly rounded reciprocal and a quotient within one ulp; checked
+// against x / 6.0 on 4.2e8 values on the host); halving is exact.
+__device__ __forceinline__ double div6(double x) {
+  const double third = 1.0 / 3.0;
+  const double q0 = x * third;
+  const double r = fma(-q0, 3.0, x);
+  return 0.5 * fma(r, third, q0);
 }
 
 // MIXED: dgamma (FP32) and theta (FP64) with the expansion of the header.
@@ -452,26 +492,38 @@ struct MixedK {
 // The stage phase is the step's initial phase plus dth (the phase advance
 // within the step, |dth| << 1): harmonic h takes sin and cos of its reduced
 // initial phase r0[i] plus h * dth in FP32.
-template <int NF>
+template <bool ACC, int NF>
 __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int nf, float dg,
                                       const float* r0, float dth, float db0, float ez,
                                       const float* rr, const float* ri, float& k2gg,
                                       float& k2pp) {
-  float cre = 0.0f, cim = 0.0f;
+  float cre, cim;
 #pragma unroll
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
     const float a = r0[i] + float(p.f[i].rharm) * dth;
     const float s = __sinf(a), c = __cosf(a);
-    cre += rr[i] * c + ri[i] * s;
-    cim += ri[i] * c - rr[i] * s;
+    if (i == 0) {
+      cre = rr[i] * c + ri[i] * s;
+      cim = ri[i] * c - rr[i] * s;
+    } else {
+      cre += rr[i] * c + ri[i] * s;
+      cim += ri[i] * c - rr[i] * s;
+    }
   }
   const float db = db0 + m.ztemp1 * cre;
   const float d = dg * m.inv_g0;
   const float inv = __builtin_amdgcn_rcpf(1.0f + d);
   const float eta = (db * m.inv_b0 - d * (2.0f + d)) * inv * inv;
-  k2pp += -(m.c0 + eta * (m.k1 + m.k2 * eta));
-  k2gg += cim * m.inv_g0 * inv - ez;
+  const float dpp = -(m.c0 + eta * (m.k1 + m.k2 * eta));
+  const float dgg = cim * m.inv_g0 * inv - ez;
+  if (ACC) {
+    k2pp += dpp;
+    k2gg += dgg;
+  } else {
+    k2pp = dpp;
+    k2gg = dgg;
+  }
 }
 
 // RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
@@ -502,23 +554,21 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
     if (NF == 0 && i >= nf) break;
     sincos(p.f[i].rharm * t0, &bs[i], &bc[i]);
   }
-  double k2gg = 0.0, k2pp = 0.0, k3gg, k3pp;
-  ode64<NF>(p, nf, g, t, 0.0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
+  double k2gg, k2pp, k3gg, k3pp;
+  ode64<false, NF>(p, nf, g, t, 0.0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   double stpz = 0.5 * p.delz;
   g += stpz * k2gg;
   t += stpz * k2pp;
   k3gg = k2gg;
   k3pp = k2pp;
-  k2gg = 0.0;
-  k2pp = 0.0;
-  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<false, NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k2gg - k3gg);
   t += stpz * (k2pp - k3pp);
-  k3gg /= 6.0;
-  k3pp /= 6.0;
+  k3gg = div6(k3gg);
+  k3pp = div6(k3pp);
   k2gg *= -0.5;
   k2pp *= -0.5;
-  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<true, NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   stpz = p.delz;
   g += stpz * k2gg;
   t += stpz * k2pp;
@@ -526,9 +576,9 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
   k3pp -= k2pp;
   k2gg *= 2.0;
   k2pp *= 2.0;
-  ode64<NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
-  g += stpz * (k3gg + k2gg / 6.0);
-  t += stpz * (k3pp + k2pp / 6.0);
+  ode64<true, NF>(p, nf, g, t, t - t0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
+  g += stpz * (k3gg + div6(k2gg));
+  t += stpz * (k3pp + div6(k2pp));
   gamma = g;
   theta = t;
   // hand the first field's initial phase factors on to the deposition and
@@ -572,23 +622,21 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   }
   const float h = float(p.delz);
   float t = 0.0f;  // phase advance within the step
-  float k2gg = 0.0f, k2pp = 0.0f, k3gg, k3pp;
-  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
+  float k2gg, k2pp, k3gg, k3pp;
+  ode32<false, NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   float stpz = 0.5f * h;
   g += stpz * k2gg;
   t += stpz * k2pp;
   k3gg = k2gg;
   k3pp = k2pp;
-  k2gg = 0.0f;
-  k2pp = 0.0f;
-  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<false, NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k2gg - k3gg);
   t += stpz * (k2pp - k3pp);
   k3gg *= (1.0f / 6.0f);
   k3pp *= (1.0f / 6.0f);
   k2gg *= -0.5f;
   k2pp *= -0.5f;
-  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<true, NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   stpz = h;
   g += stpz * k2gg;
   t += stpz * k2pp;
@@ -596,7 +644,7 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   k3pp -= k2pp;
   k2gg *= 2.0f;
   k2pp *= 2.0f;
-  ode32<NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
+  ode32<true, NF>(p, m, nf, g, r0, t, db0, ez, rr, ri, k2gg, k2pp);
   g += stpz * (k3gg + k2gg * (1.0f / 6.0f));
   t += stpz * (k3pp + k2pp * (1.0f / 6.0f));
   dgamma = g;

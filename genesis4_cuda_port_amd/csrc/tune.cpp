@@ -6,10 +6,11 @@
 // many planes fill the GPU either way, a handful of planes leaves it idle
 // unless the lines of one plane are spread out.  The tuner measures instead
 // of guessing: the first kReps * nvar calls of a (family, shape) run the
-// variants in turn between two events; once every trial's end event has
-// completed (checked with hipEventQuery, so the host never waits) the
-// variant with the smallest best time is fixed for that shape, for this
-// context and for contexts created later in the process on the same device.
+// variants in turn between two events; the call after the last trial waits
+// for their end events (outside a graph capture; inside one the events are
+// only queried) and fixes the variant with the smallest best time for that
+// shape, for this context and for contexts created later in the process on
+// the same device.
 #include <algorithm>
 #include <limits>
 #include <mutex>
@@ -63,8 +64,13 @@ int Context::tune_pick(int family, const std::vector<int>& shape, int nvar, hipE
     *b = t.b;
     return t.variant;
   }
-  // all trials issued: decide once they have all completed
+  // all trials issued: decide once they have all completed.  Outside a
+  // capture the host waits for them here, once per shape (a few launches'
+  // time), rather than running variant 0 until they happen to be done: the
+  // host may be many steps ahead of the device, and variant 0 can be several
+  // times slower than the best on shapes with few planes.
   for (const Tuner::Trial& t : e.trials) {
+    if (!capturing) G4_HIP(hipEventSynchronize(t.b));
     const hipError_t q = hipEventQuery(t.b);
     if (q == hipErrorNotReady) return 0;  // provisional: variant 0
     G4_HIP(q);

@@ -1,9 +1,10 @@
 """GPU: the request loop (g4h_server_*).
 
 A mix of steady-state requests (batched together) and time-dependent
-requests is submitted at once to a server with three stream lanes; every
-result must equal the same request run alone through g4h_run (FP64,
-1e-12 relative on slice power and bunching), the latency statistics must
+requests is submitted at once to a server with three stream lanes, in each
+precision; every result must equal the same request run alone through
+g4h_run (FP64 1e-12, MIXED 1e-6 relative on slice power and bunching: the
+batched MIXED kernels sum in a different order), the latency statistics must
 cover every request, and a request larger than the HBM budget must fail
 with G4H_ERR_OUT_OF_MEMORY without disturbing the others."""
 from __future__ import annotations
@@ -25,7 +26,7 @@ def relerr(a, b):
     return np.linalg.norm(np.ravel(a) - np.ravel(b)) / (den if den > 0 else 1.0)
 
 
-def _alone(ctx, r, steps):
+def _alone(ctx, r, steps, prec):
     """The request run alone through g4h_run, from the state the server
     builds: steady-state beams from the host loader (as g4h_batch_create),
     time-dependent ones generated on the device with the seed plane filled
@@ -45,8 +46,8 @@ def _alone(ctx, r, steps):
         re = np.zeros((1 if r["kind"] == 1 else S) * G * G)
         im = np.zeros_like(re)
     off = (np.arange(S + 1) * r["npart"]).astype(np.int32)
-    b = api.Beam(ctx, api.FP64, off, np.full(S, r["current"]), GAMMA0)
-    f = api.Field(ctx, api.FP64, S, G, GRIDMAX, LAMBDA0, LAMBDA0 * r.get("sample", 1.0))
+    b = api.Beam(ctx, prec, off, np.full(S, r["current"]), GAMMA0)
+    f = api.Field(ctx, prec, S, G, GRIDMAX, LAMBDA0, LAMBDA0 * r.get("sample", 1.0))
     if r["kind"] == 1:
         b.generate(**kw)
         f.fill_planes(re, im, 0)
@@ -64,11 +65,12 @@ def _alone(ctx, r, steps):
     return out
 
 
-def test_server_mix_matches_lone_runs(gpu_ctx):
-    print("case server mix", flush=True)
+@pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
+def test_server_mix_matches_lone_runs(gpu_ctx, prec):
+    print(f"case server mix prec={prec}", flush=True)
     steps = api.lattice_fodo(LAMBDAU, [AW], 24, 10, 3, 1.5, 2)
     taper = AW * (1 - 1e-3 * np.arange(len(steps)) / len(steps))
-    common = dict(precision=api.FP64, gridmax=GRIDMAX, gamma0=GAMMA0, lambda0=LAMBDA0,
+    common = dict(precision=prec, gridmax=GRIDMAX, gamma0=GAMMA0, lambda0=LAMBDA0,
                   espread_rel=1e-4)
     reqs = [
         dict(kind=0, solver=api.ADI, nslice=1, npart=512, ngrid=33, current=3000.0, seed=21,
@@ -98,8 +100,9 @@ def test_server_mix_matches_lone_runs(gpu_ctx):
         srv.release(i)
     srv.close()
     for k, (r, (braw, fraw)) in enumerate(zip(reqs, results)):
-        lb, lf = _alone(gpu_ctx, r, steps)
+        lb, lf = _alone(gpu_ctx, r, steps, prec)
         assert braw.shape == lb.shape and fraw.shape == lf.shape
         eb, ef = relerr(braw, lb), relerr(fraw, lf)
         print(f"  request {k}: beam {eb:.3e} field {ef:.3e}", flush=True)
-        assert eb < 1e-12 and ef < 1e-12, k
+        tol = 1e-12 if prec == api.FP64 else 1e-6
+        assert eb < tol and ef < tol, k
