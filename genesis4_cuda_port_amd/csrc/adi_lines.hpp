@@ -401,5 +401,195 @@ __device__ __forceinline__ double wave_sum64(double v) {
   return v;
 }
 
+
+// ---- segmented sweeps -------------------------------------------------------
+// The sweeps above run each line's recurrence in one lane from end to end:
+// a plane of G lines is ceil(G / 64) waves, each waiting on its chain of G
+// elements, and the forward values cross HBM (an FP64 plane written and read
+// back).  Here a workgroup of kSegs waves owns 64 lines of a plane and wave j
+// the elements [j m, j m + m) of each (m = ceil(G / kSegs) <= kSegMax), kept
+// in registers from the load to the store.  Both Thomas recurrences are
+// affine in the value entering a segment,
+//   forward   y_k = alpha_k + beta_k  y_(s-1),   alpha: y_(s-1) = 0,
+//   backward  u_k = gamma_k + delta_k u_e,       gamma: u_e = 0,
+// where beta_k = prod_(i=s..k) (-c_i cbet_i) and delta_k = prod_(i=k+1..e)
+// (-cwet_i) do not depend on the line (host tables, zero past the grid), so
+// each wave runs its segment with nothing entering, the values entering
+// every segment follow from the segments' end values in one short pass
+// through LDS (j steps), and one more pass over the registers corrects them.
+// The factors satisfy |c cbet|, |cwet| < 1 (the system is diagonally
+// dominant), so the products decay and the corrections add no error growth;
+// the result differs from the single-chain sweep by rounding only.
+constexpr int kSegs = 16;    // waves per workgroup = segments per line
+constexpr int kSegMax = 16;  // longest segment: G <= kSegs * kSegMax
+
+struct SegTables {
+  const double2* cl;  // c
+  const double2* cb;  // cbet
+  const double2* cw;  // cwet
+  const double2* be;  // beta, per element of its segment
+  const double2* de;  // delta, per element of its segment
+};
+
+// Segments of the workgroup's LPW lines (blk) of one plane, kSegs * LPW
+// threads (thread j * LPW + l: segment j of line l): XDIR sweeps along rows
+// (step A, lines are rows), else along columns (step B).  sA: LDS
+// [kSegs][LPW]; with STATS the field sums of the block go to part[0..6] and
+// the centre cell to centre[0..1] (red: LDS [waves][8]).
+template <class P, int LPW, bool XDIR, bool STATS>
+__device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
+                                          typename A2<P>::T* uout, int G, int m, double cs,
+                                          SegTables tb, const typename A2<P>::T* sp, int4 rc,
+                                          int blk, double2* sA, double* red, double* part,
+                                          double* centre) {
+#pragma clang fp contract(off)
+  using T2 = typename A2<P>::T;
+  constexpr int E = sizeof(T2);
+  constexpr int M = kSegMax;
+  constexpr int NW = kSegs * LPW / kLane;  // waves of the workgroup
+  // edge and source loads ahead (fewer with the field sums, which hold
+  // their own registers: 128 per lane fit four waves per SIMD)
+  constexpr int CH = (sizeof(P) == 8 ? 2 : 4) / (STATS ? 2 : 1);
+  constexpr int AIN = XDIR ? 0 : kNT;  // the y sweep reads its lines once
+  const int t = threadIdx.x & (kLane - 1);
+  const int l = threadIdx.x % LPW, j = threadIdx.x / LPW;
+  const int L = blk * LPW + l;
+  const bool act = L < G;
+  const int eL = l == 0 ? L - 1 : (l == LPW - 1 ? L + 1 : -1);
+  const bool eact = eL >= 0 && eL < G;
+  const bool sline =
+      sp != nullptr && act && (XDIR ? (L >= rc.y && L < rc.w) : (L >= rc.x && L < rc.z));
+  const int klo = XDIR ? rc.x : rc.y, khi = XDIR ? rc.z : rc.w;
+  const int s = j * m;
+  const int n = max(0, min(G, s + m) - s);  // elements of this segment (uniform)
+  const Rsrc ru = plane_rsrc(uin, G, E);
+  const Rsrc rs = plane_rsrc(sp, G, E);
+  const Rsrc ro = plane_rsrc(uout, G, E);
+  auto off = [&](bool ok, int line, int k) {
+    return XDIR ? cell_off(ok, line, k, G, E) : cell_off(ok, k, line, G, E);
+  };
+  // 1. the lane's own segment, all of it in flight at once
+  double2 y[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) y[c] = to_d(bload<AIN, T2>(ru, off(act && c < n, L, s + c)));
+  // the edge lanes' neighbour lines and the source, CH elements ahead
+  T2 ne[CH], ns[CH];
+  auto load_es = [&](int c0) {
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int c = c0 + q, k = s + c;
+      const bool in = c < n;
+      ne[q] = bload<0, T2>(ru, off(eact && in, eL, k));
+      ns[q] = bload<0, T2>(rs, off(sline && in && k >= klo && k < khi, L, k));
+    }
+  };
+  load_es(0);
+  double2 a = make_double2(0.0, 0.0);  // alpha of the previous element
+#pragma unroll
+  for (int c0 = 0; c0 < M; c0 += CH) {
+    double2 e[CH], sv[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      e[q] = to_d(ne[q]);
+      sv[q] = to_d(ns[q]);
+    }
+    if (c0 + CH < M) load_es(c0 + CH);
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int c = c0 + q, k = s + c;
+      if (c < n) {  // uniform
+        const double2 cen = y[c];
+        double2 lo = from_lane_below(cen, e[q]);  // line L - 1
+        double2 hi = from_lane_above(cen, e[q]);  // line L + 1
+        if constexpr (LPW < kLane) {  // the wave holds several segments' lines
+          lo = l == 0 ? e[q] : lo;
+          hi = l == LPW - 1 ? e[q] : hi;
+        }
+        double2 r = rhs_of(cen, hi, lo, cs);
+        const bool sin_ = sline && k >= klo && k < khi;
+        const double2 rs_ = zadd(r, sv[q]);
+        r.x = sin_ ? rs_.x : r.x;
+        r.y = sin_ ? rs_.y : r.y;
+        a = zmul(zsub(r, zmul(tb.cl[k], a)), tb.cb[k]);
+        y[c] = a;
+      }
+    }
+  }
+  // 2. the forward value entering the segment, from the end values of the
+  //    segments before it
+  sA[j * LPW + l] = a;
+  __syncthreads();
+  double2 yin = make_double2(0.0, 0.0);
+  for (int i = 0; i < j; ++i) {
+    const int si = i * m, ei = min(G, si + m);
+    if (ei > si) yin = zadd(sA[i * LPW + l], zmul(tb.be[ei - 1], yin));
+  }
+  // 3. forward values, then the back substitution inside the segment with
+  //    zero entering from the right
+#pragma unroll
+  for (int c = 0; c < M; ++c)
+    if (c < n) y[c] = zadd(y[c], zmul(tb.be[s + c], yin));
+  double2 g = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int c = M - 1; c >= 0; --c) {
+    if (c < n) {
+      if (c < n - 1) y[c] = zsub(y[c], zmul(tb.cw[s + c + 1], g));
+      g = y[c];
+    }
+  }
+  // 4. the value entering from the right, from the segments after it
+  __syncthreads();  // every wave has read the forward end values
+  sA[j * LPW + l] = g;
+  __syncthreads();
+  double2 uin_r = make_double2(0.0, 0.0);
+  for (int i = kSegs - 1; i > j; --i) {
+    const int si = i * m;
+    if (si < G) uin_r = zadd(sA[i * LPW + l], zmul(tb.de[si], uin_r));
+  }
+  // 5. final values
+  PlaneSums sums;
+  sums.clear();
+  const double shift = -0.5 * static_cast<double>(G - 1);
+  const int64_t icenter = (static_cast<int64_t>(G) * G - 1) / 2;
+  const int crow = static_cast<int>(icenter / G), ccol = static_cast<int>(icenter % G);
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    if (c < n) {
+      const int k = s + c;
+      const double2 uu = zadd(y[c], zmul(tb.de[k], uin_r));
+      const T2 o = from_d<P>(uu);
+      bstore<AIN>(ro, off(act, L, k), o);
+      if constexpr (STATS) {
+        const double re = o.x, im = o.y;
+        const double wei = re * re + im * im;
+        const double dx = L + shift, dy = k + shift;  // column L, row k
+        const double v[7] = {wei, dx * wei, dx * dx * wei, dy * wei, dy * dy * wei, re, im};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          const double nq = sums.acc[q] + v[q];
+          sums.acc[q] = act ? nq : sums.acc[q];
+        }
+        if (k == crow && L == ccol) {
+          centre[0] = re;
+          centre[1] = im;
+        }
+      }
+    }
+  }
+  if constexpr (STATS) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const double v = wave_sum64(sums.acc[q]);
+      if (t == 0) red[(threadIdx.x / kLane) * 8 + q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double v = 0.0;
+      for (int i = 0; i < NW; ++i) v += red[i * 8 + threadIdx.x];
+      part[threadIdx.x] = v;
+    }
+  }
+}
+
 }  // namespace adi
 }  // namespace g4

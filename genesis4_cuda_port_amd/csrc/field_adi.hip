@@ -364,6 +364,68 @@ __global__ __launch_bounds__(kLane) void adi_y_lane_kernel(typename A2<P>::T* u,
   }
 }
 
+// Segmented sweeps (adi_lines.hpp): a workgroup of kSegs waves per 64 lines
+// of a plane, every element in registers between its load and its store,
+// the tables of the step (c, cbet, cwet, beta, delta) staged in LDS.  For
+// G <= kSegs * kSegMax.
+template <class P, int LPW>
+__global__ __launch_bounds__(adi::kSegs * LPW, 4) void adi_x_seg_kernel(
+    const typename A2<P>::T* u, typename A2<P>::T* ustar, int nslice, int G, int m, double cs,
+    const double2* __restrict__ tab, const typename A2<P>::T* src, const int4* rect, int first,
+    int add_src) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 lds[];
+  for (int k = threadIdx.x; k < 5 * G; k += blockDim.x) lds[k] = tab[k];
+  __syncthreads();
+  const int p = blockIdx.x;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const adi::SegTables tb{lds, lds + G, lds + 2 * G, lds + 3 * G, lds + 4 * G};
+  adi::seg_sweep<P, LPW, true, false>(u + p * plane, ustar + p * plane, G, m, cs, tb, sp, rc,
+                                 blockIdx.y, lds + 5 * G, nullptr, nullptr, nullptr);
+}
+
+// ... and the y sweep, with the block's field sums in the layout of the
+// wave-per-block variants (partials[(p * nblk + b) * 8 + k], centres at the
+// tail).
+template <class P, int LPW>
+__global__ __launch_bounds__(adi::kSegs * LPW, 4) void adi_y_seg_kernel(
+    typename A2<P>::T* u, const typename A2<P>::T* ustar, int nslice, int G, int m, double cs,
+    const double2* __restrict__ tab, const typename A2<P>::T* src, const int4* rect, int first,
+    int add_src, double* partials) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 lds[];
+  __shared__ double red[adi::kSegs * LPW / kLane * 8];
+  for (int k = threadIdx.x; k < 5 * G; k += blockDim.x) lds[k] = tab[k];
+  __syncthreads();
+  const int p = blockIdx.x;
+  const int nblk = gridDim.y;
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  const int s = ((p - first) % nslice + nslice) % nslice;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = rect[s];
+    sp = src + s * plane;
+  }
+  const adi::SegTables tb{lds, lds + G, lds + 2 * G, lds + 3 * G, lds + 4 * G};
+  if (partials) {
+    double* part = partials + (static_cast<int64_t>(p) * nblk + blockIdx.y) * 8;
+    double* centre = partials + static_cast<int64_t>(nslice) * nblk * 8 + 2 * p;
+    adi::seg_sweep<P, LPW, false, true>(ustar + p * plane, u + p * plane, G, m, cs, tb, sp, rc,
+                                   blockIdx.y, lds + 5 * G, red, part, centre);
+  } else {
+    adi::seg_sweep<P, LPW, false, false>(ustar + p * plane, u + p * plane, G, m, cs, tb, sp, rc,
+                                    blockIdx.y, lds + 5 * G, nullptr, nullptr, nullptr);
+  }
+}
+
 // The x sweep's output of the wave-per-block variants (the y sweep's input),
 // one plane per ring slot in the field's storage type.
 void* adi_ustar(Field* f) {
@@ -382,7 +444,6 @@ void adi_step(Field* f, bool source_on, bool stats) {
   const int G = f->ngrid;
   const int nt = ((G + 63) / 64) * 64;
   const double cs = 0.25 * f->adi_delz / (f->xks * f->dgrid * f->dgrid);
-  if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
   T2* u = static_cast<T2*>(f->data());
   const T2* src = static_cast<const T2*>(f->src());
   const size_t ldsx = sizeof(double2) * (3 * static_cast<size_t>(G) + (G + 2) * static_cast<size_t>(kB));
@@ -394,8 +455,50 @@ void adi_step(Field* f, bool source_on, bool stats) {
   // the loads 8 or 16 elements ahead of the recurrence (16 holds more loads
   // in flight, which is what a sweep over few planes waits on, for more
   // registers).  Tuned per shape; all three give bit-identical fields.
+  //
+  // Variant 3, the segmented sweeps, holds every element in registers (no
+  // FP64 forward-value plane) and spreads a line over kSegs waves; it rounds
+  // differently from the single-chain variants, so it is not a tuning
+  // candidate: it is what runs for G <= kSegs * kSegMax unless a variant is
+  // pinned, and the tuner chooses among variants 0-2 for larger grids only.
+  const bool segable = G <= adi::kSegs * adi::kSegMax;
+  const int forced = f->ctx->tuner.forced[TUNE_ADI];
+  if (segable && (forced < 0 || forced == 3)) {
+    auto seg = [&](auto lpw) {
+      constexpr int LPW = decltype(lpw)::value;
+      const int nblk = (G + LPW - 1) / LPW;
+      const int m = (G + adi::kSegs - 1) / adi::kSegs;
+      double* partials = nullptr;
+      if (stats) {
+        const size_t need = static_cast<size_t>(f->nslice) * nblk * 8 + 2 * static_cast<size_t>(f->nslice);
+        if (f->col_partials.size() < need) f->col_partials.alloc(f->ctx, need);
+        partials = f->col_partials.get();
+      }
+      const size_t lds = sizeof(double2) * (5 * static_cast<size_t>(G) + adi::kSegs * LPW);
+      T2* ustar = static_cast<T2*>(adi_ustar(f));
+      f->ctx->launch(G4H_K_ADI_X, [&] {
+        hipLaunchKernelGGL((adi_x_seg_kernel<P, LPW>), dim3(f->nslice, nblk),
+                           dim3(adi::kSegs * LPW), lds, f->ctx->stream, u, ustar, f->nslice, G,
+                           m, cs, f->adi_c, src, f->src_rect.get(), f->first, add);
+      });
+      f->ctx->launch(G4H_K_ADI_Y, [&] {
+        hipLaunchKernelGGL((adi_y_seg_kernel<P, LPW>), dim3(f->nslice, nblk),
+                           dim3(adi::kSegs * LPW), lds, f->ctx->stream, u, ustar, f->nslice, G,
+                           m, cs, f->adi_c, src, f->src_rect.get(), f->first, add, partials);
+      });
+      if (stats) field_reduce_partials(f, partials, nblk);
+    };
+    // 64 lines per workgroup (one wave per segment): with 32 (two segments
+    // per wave) the segment bounds are no longer wave-uniform, the element
+    // loops diverge and spill, and the time-dependent G = 151 sweeps ran
+    // 1.5x slower (0.52 against 0.35 ms per x sweep)
+    seg(std::integral_constant<int, 64>{});
+    return;
+  }
+  if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
   TuneScope tune(f->ctx, TUNE_ADI, {G, f->nslice, static_cast<int>(f->prec)}, 3);
-  const int variant = (tune.variant() == 0 && G > 360) ? 1 : tune.variant();
+  const int picked = forced == 3 ? 1 : tune.variant();  // 3 pinned on a grid it cannot take
+  const int variant = (picked == 0 && G > 360) ? 1 : picked;
   if (variant >= 1) {
     const int nblk = (G + kLane - 1) / kLane;
     double* partials = nullptr;
@@ -476,15 +579,35 @@ void adi_prepare(Field* f, double delz) {
       cwet[i] = w1[i - 1] * cbet[i - 1];
       cbet[i] = cd(1.0, 0.0) / (w2[i] - c[i] * cwet[i]);
     }
+    // the segmented sweeps' products over each segment of m elements:
+    // beta_k = prod_(i=s..k) (-c_i cbet_i), delta_k = prod_(i=k+1..e) (-cwet_i),
+    // delta = 0 in the segment that ends the line
+    const int m = (G + adi::kSegs - 1) / adi::kSegs;
+    std::vector<cd> beta(G), delta(G);
+    for (int s0 = 0; s0 < G; s0 += m) {
+      const int e0 = std::min(G, s0 + m);
+      cd b(1.0, 0.0);
+      for (int k = s0; k < e0; ++k) {
+        b = -c[k] * cbet[k] * b;
+        beta[k] = b;
+      }
+      cd d(e0 < G ? -cwet[e0] : cd(0.0, 0.0));
+      for (int k = e0 - 1; k >= s0; --k) {
+        delta[k] = d;
+        if (k > s0) d = -cwet[k] * d;
+      }
+    }
     Field::DelzTable t;
-    t.host.resize(3 * static_cast<size_t>(G));
+    t.host.resize(5 * static_cast<size_t>(G));
     for (int i = 0; i < G; ++i) {
       t.host[i] = make_double2(c[i].real(), c[i].imag());
       t.host[G + i] = make_double2(cbet[i].real(), cbet[i].imag());
       t.host[2 * G + i] = make_double2(cwet[i].real(), cwet[i].imag());
+      t.host[3 * G + i] = make_double2(beta[i].real(), beta[i].imag());
+      t.host[4 * G + i] = make_double2(delta[i].real(), delta[i].imag());
     }
-    t.dev.alloc(f->ctx, 3 * static_cast<size_t>(G));
-    G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * 3 * G,
+    t.dev.alloc(f->ctx, 5 * static_cast<size_t>(G));
+    G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * 5 * G,
                           hipMemcpyHostToDevice, f->ctx->stream));
     it = f->delz_tables.emplace(key, std::move(t)).first;
   }

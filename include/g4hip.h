@@ -62,7 +62,11 @@ g4h_status g4h_context_memory(const g4h_context* ctx, uint64_t* bytes_in_use);
  * shape each).  Families: */
 #define G4H_TUNE_ADI 0   /* 0: workgroup per plane (ngrid <= 360, else as 1),
                             1 / 2: wave per 64 lines, loads 8 / 16 elements
-                            ahead; bit-identical results */
+                            ahead; bit-identical results.  3: segmented
+                            sweeps (16 waves per 64 lines, no forward-value
+                            plane; ngrid <= 256, else as 1), results agree
+                            with 0-2 to round-off.  Unpinned, 3 runs for
+                            ngrid <= 256 and 0-2 are tuned above */
 #define G4H_TUNE_SPLIT 1 /* workgroups per slice of the particle pass (>= 1; by
                             default chosen from the slice count so that few
                             slices still fill the GPU; results agree to

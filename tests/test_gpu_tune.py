@@ -1,11 +1,15 @@
 """GPU: kernel variants and the autotuner.
 
-The three ADI variants (workgroup per plane; wave per 64 lines with 8- or
-16-element load chunks) must give bit-identical fields and field sums, each
-must match the FP64 oracle with the source on, the lane variants must cover
-grids the workgroup variant cannot hold in LDS (G > 360), and the tuner
-must settle on a variant after its trial launches without changing
-results."""
+The three single-chain ADI variants (workgroup per plane; wave per 64 lines
+with 8- or 16-element load chunks) must give bit-identical fields and field
+sums; each of them and the segmented variant 3 (16 waves per 64 lines, every
+element in registers) must match the FP64 oracle with the source on; the
+lane variants must cover grids the workgroup variant cannot hold in LDS
+(G > 360); an unpinned context runs the segmented sweeps for G <= 256
+(equal bit for bit to a run pinned to 3, and to the single-chain runs within
+round-off, FP64 1e-13 and MIXED 1e-6 on the fields and sums), and on larger
+grids the tuner must settle on one of 0-2 after its trial launches without
+changing results."""
 from __future__ import annotations
 
 import numpy as np
@@ -56,7 +60,9 @@ def test_adi_variants_bitexact(prec, ngrid):
 
 
 @pytest.mark.parametrize("variant,ngrid", [(0, 33), (1, 33), (1, 64), (1, 65), (1, 400),
-                                            (2, 33), (2, 65), (2, 400), (1, 1024), (2, 1024)])
+                                            (2, 33), (2, 65), (2, 400), (1, 1024), (2, 1024),
+                                            (3, 9), (3, 17), (3, 33), (3, 64), (3, 65), (3, 101),
+                                            (3, 151), (3, 255), (3, 256), (3, 400)])
 def test_adi_variant_oracle_fp64(variant, ngrid):
     """Deposition + ADI step with the source on, each variant pinned."""
     print(f"case adi_variant_oracle variant={variant} ngrid={ngrid}", flush=True)
@@ -87,11 +93,32 @@ def test_adi_variant_oracle_fp64(variant, ngrid):
         b.close()
 
 
+@pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
+@pytest.mark.parametrize("ngrid", [45, 151, 256])
+def test_segmented_is_the_default(prec, ngrid):
+    """Unpinned on G <= 256: the segmented sweeps, equal bit for bit to a
+    run pinned to variant 3 and within round-off of the single-chain
+    variant 1 (fields and field sums of every record)."""
+    print(f"case segmented_default prec={prec} ngrid={ngrid}", flush=True)
+    c = cases.make_case(nslice=4, npart=64, ngrid=ngrid, seed_power=1e6, current=0.0, nsteps=6)
+    re, im = noisy_field(c, scale=0.1)
+    d = _run_records(prec, -1, c, re, im, 6)
+    p3 = _run_records(prec, 3, c, re, im, 6)
+    p1 = _run_records(prec, 1, c, re, im, 6)
+    for x, y in zip(d, p3):
+        assert np.array_equal(x, y)
+    tol = 1e-13 if prec == api.FP64 else 1e-6
+    e = [relerr(x, y) for x, y in zip(d, p1)]
+    print(f"  vs single chain: field {e[0]:.2e} {e[1]:.2e} sums {e[2]:.2e}", flush=True)
+    assert max(e) < tol
+
+
 def test_tuner_decides():
-    """Unpinned, the tuner tries both variants (two launches each) and then
-    keeps one; the records equal a run pinned to variant 0."""
+    """Unpinned on a grid above the segmented sweeps' limit, the tuner tries
+    the variants (three launches each) and then keeps one; the records equal
+    a run pinned to variant 0."""
     print("case tuner_decides", flush=True)
-    c = cases.make_case(nslice=4, npart=64, ngrid=45, seed_power=1e6, current=0.0, nsteps=12)
+    c = cases.make_case(nslice=4, npart=64, ngrid=300, seed_power=1e6, current=0.0, nsteps=12)
     re, im = noisy_field(c, scale=0.1)
     ref = _run_records(api.FP64, 0, c, re, im, 12)
     with api.Context(0) as ctx:
