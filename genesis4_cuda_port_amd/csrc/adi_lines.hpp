@@ -436,12 +436,16 @@ struct SegTables {
 // (step A, lines are rows), else along columns (step B).  sA: LDS
 // [kSegs][LPW]; with STATS the field sums of the block go to part[0..6] and
 // the centre cell to centre[0..1] (red: LDS [waves][8]).
+// sclear: the source plane sp again, whose cells this block read are then
+// set to zero (the step's last reader of the source clears it for the next
+// deposition), or nullptr.
 template <class P, int LPW, bool XDIR, bool STATS>
 __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
                                           typename A2<P>::T* uout, int G, int m, double cs,
                                           SegTables tb, const typename A2<P>::T* sp, int4 rc,
                                           int blk, double2* sA, double* red, double* part,
-                                          double* centre) {
+                                          double* centre,
+                                          typename A2<P>::T* sclear = nullptr) {
 #pragma clang fp contract(off)
   using T2 = typename A2<P>::T;
   constexpr int E = sizeof(T2);
@@ -513,6 +517,17 @@ __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
         a = zmul(zsub(r, zmul(tb.cl[k], a)), tb.cb[k]);
         y[c] = a;
       }
+    }
+  }
+  if (sclear) {
+    const Rsrc rw = plane_rsrc(sclear, G, E);
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      const int k = s + c;
+      bstore<0>(rw, off(sline && c < n && k >= klo && k < khi, L, k), z);
     }
   }
   // 2. the forward value entering the segment, from the end values of the

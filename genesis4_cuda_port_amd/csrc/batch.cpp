@@ -13,14 +13,21 @@
 // host from the same make_trans / make_push / make_dep as every other path,
 // for every step at creation, and kept on the device ([step][run]).
 //
-// Each group has its own context (its own stream) and each step is one
-// particle launch and one solver step per group, so the groups' launches
-// overlap on the device; g4h_batch_advance joins them into the caller's
-// stream before it returns.  ADI batches can instead run every step of
-// every run in ONE launch (G4H_TUNE_BATCH = 1; launch_batch_persistent: one
-// workgroup per run loops over the steps, the per-step Thomas factors of each
-// group and the record index of each step in device arrays built here).  On
-// the taper scan that launch is slower than the per-step launches (the
+// ADI batches on grids up to 256 (the segmented sweeps) run each step of
+// EVERY run as one launch per kernel on the batch's stream (G4H_TUNE_BATCH =
+// 0, the default): the particle passes (one workgroup per run, its group's
+// beam slice and parameter entry), the x and the y sweep (one workgroup per
+// 64-line block of every run's plane, its group's tables of the step), and on
+// output steps one kernel writing every run's field record.  Three or four
+// launches per step, whatever the number of grids: per-group launches on
+// per-group streams (G4H_TUNE_BATCH = 2, and what other batches run) spent
+// ~16 host launches per step on the 96-run taper scan, whose host time then
+// bounded the step.  Groups keep their own contexts (their own streams) for
+// creation, records and downloads; g4h_batch_advance orders the batch's
+// stream after them and them after it.  ADI batches can also run every step
+// of every run in ONE launch (G4H_TUNE_BATCH = 1; launch_batch_persistent:
+// one workgroup per run loops over the steps with the single-chain sweeps);
+// on the taper scan that launch is slower than the per-step launches (the
 // sweeps' recurrences of a plane run on one CU with their memory latency
 // exposed), so it is not the default.
 //
@@ -77,9 +84,19 @@ struct g4h_batch {
   std::vector<std::pair<int, int>> where;  // run -> (group, slice in group)
   int done = 0, nrec = 0, cap = 0;
   std::vector<int32_t> rec_of_step;  // record index written by each step, or -1
-  // one-launch path
-  bool persistent = false;
+  // one-launch path, and the per-step launches of every run (grid)
+  bool persistent = false, grid = false;
   int win_cells = 0, gmax = 0;
+  DevBuf<BatchSegDesc> dblocks;
+  int nblocks = 0;
+  std::vector<uint8_t> src_all;  // per step: the source is on in every group
+  std::vector<BatchRunDesc> hdesc;  // host copy of druns
+  DevBuf<int2> dparts;              // (run, part) of each particle workgroup
+  int nparts = 0;
+  DevBuf<double2> swin;             // split scratch of every run
+  DevBuf<double> sred;
+  DevBuf<int4> sfb;
+  DevBuf<int32_t> scnt;
   DevBuf<BatchRunDesc> druns;
   DevBuf<int32_t> drec;
   std::vector<DevBuf<const double2*>> dadi;  // per group: factors of each step
@@ -87,6 +104,12 @@ struct g4h_batch {
   hipEvent_t pdone = nullptr;
 
   ~g4h_batch() {
+    dblocks.release();
+    dparts.release();
+    swin.release();
+    sred.release();
+    sfb.release();
+    scnt.release();
     druns.release();
     drec.release();
     dadi.clear();
@@ -180,6 +203,17 @@ struct g4h_batch {
   // descriptors and record indices on the device.  Returns false (and the
   // batch keeps the per-step launches) when the kernel does not fit.
   bool setup_persistent(Context* c) {
+    if (!lengths_fit()) return false;
+    if (!batch_persistent_fits(c, prec, nharm, win_cells, gmax)) return false;
+    build_run_descs(c);
+    G4_HIP(hipEventCreateWithFlags(&pdone, hipEventDisableTiming));
+    persistent = true;
+    return true;
+  }
+
+  // Whether every distinct step length's solver tables can stay resident
+  // (at most Field::kMaxDelzTables); sets the largest grid and window.
+  bool lengths_fit() {
     if (solver != G4H_SOLVER_ADI) return false;
     std::vector<double> lens;
     for (const g4h_step& st : steps) lens.push_back(st.delz);
@@ -191,9 +225,102 @@ struct g4h_batch {
       gm = std::max(gm, g.G);
       for (const ParticleParams& p : g.head) wc = std::max(wc, p.dep.window_cap);
     }
-    if (!batch_persistent_fits(c, prec, nharm, wc, gm)) return false;
     win_cells = wc;
     gmax = gm;
+    return true;
+  }
+
+  // The per-step launches of every run: run descriptors as for the one-launch
+  // path, plus each run's x-sweep output and partials, and one descriptor per
+  // 64-line block of every plane.  Returns false (the batch keeps the
+  // per-group launches) for grids above the segmented sweeps' limit or when
+  // the source is on in some groups and off in others at some step.
+  bool setup_grid(Context* c) {
+    if (!lengths_fit() || gmax > 256) return false;
+    const int ns = static_cast<int>(steps.size());
+    src_all.assign(static_cast<size_t>(ns), 0);
+    for (int k = 0; k < ns; ++k) {
+      const uint8_t a = groups[0].src_on[static_cast<size_t>(k)];
+      for (const Group& g : groups)
+        if (g.src_on[static_cast<size_t>(k)] != a) return false;
+      src_all[static_cast<size_t>(k)] = a;
+    }
+    build_run_descs(c);
+    // Particle workgroups: a run of n particles takes K = ceil(n / 2048)
+    // parts (at most 16), eight iterations of the 256-thread loop each, so
+    // that a batch of runs of a few thousand particles fills the GPU.
+    std::vector<int2> parts;
+    std::vector<int> kof(hdesc.size());
+    size_t ktot = 0;
+    for (size_t r = 0; r < hdesc.size(); ++r) {
+      const auto [gi, si] = where[r];
+      const Beam* bm = groups[static_cast<size_t>(gi)].beam;
+      const int np = bm->off_h[static_cast<size_t>(si) + 1] - bm->off_h[static_cast<size_t>(si)];
+      kof[r] = std::max(1, std::min(16, (np + 2047) / 2048));
+      for (int q = 0; q < kof[r]; ++q) parts.push_back(make_int2(static_cast<int>(r), q));
+      ktot += static_cast<size_t>(kof[r]);
+    }
+    const size_t red = batch_split_red_stride();
+    swin.alloc(c, ktot * static_cast<size_t>(win_cells));
+    sred.alloc(c, ktot * red);
+    sfb.alloc(c, ktot);
+    scnt.alloc(c, hdesc.size());
+    G4_HIP(hipMemsetAsync(scnt.get(), 0, sizeof(int32_t) * hdesc.size(), c->stream));
+    size_t at = 0;
+    for (size_t r = 0; r < hdesc.size(); ++r) {
+      BatchRunDesc& d = hdesc[r];
+      d.K = kof[r];
+      d.swin = swin.get() + at * static_cast<size_t>(win_cells);
+      d.sred = sred.get() + at * red;
+      d.sfb = sfb.get() + at;
+      d.scnt = scnt.get() + r;
+      at += static_cast<size_t>(kof[r]);
+    }
+    G4_HIP(hipMemcpyAsync(druns.get(), hdesc.data(), sizeof(BatchRunDesc) * hdesc.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    nparts = static_cast<int>(parts.size());
+    dparts.alloc(c, parts.size());
+    G4_HIP(hipMemcpyAsync(dparts.get(), parts.data(), sizeof(int2) * parts.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    std::vector<BatchSegDesc> blocks;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      Group& g = groups[gi];
+      Field* f = g.field;
+      const int n = static_cast<int>(g.runs.size());
+      const int nblk = (g.G + 63) / 64;
+      for (int i = 0; i < n; ++i)
+        for (int b = 0; b < nblk; ++b) {
+          BatchSegDesc d{};
+          d.u = f->data();
+          d.ustar = field_adi_ustar(f);
+          d.src = f->src();
+          d.rect = f->src_rect.get();
+          d.partials = f->col_partials.get();
+          d.adi = dadi[gi].get();
+          d.cs = dcs[gi].get();
+          d.G = g.G;
+          d.nblk = nblk;
+          d.nrun = n;
+          d.slice = i;
+          d.blk = b;
+          blocks.push_back(d);
+        }
+    }
+    nblocks = static_cast<int>(blocks.size());
+    dblocks.alloc(c, blocks.size());
+    G4_HIP(hipMemcpyAsync(dblocks.get(), blocks.data(), sizeof(BatchSegDesc) * blocks.size(),
+                          hipMemcpyHostToDevice, c->stream));
+    G4_HIP(hipStreamSynchronize(c->stream));
+    G4_HIP(hipEventCreateWithFlags(&pdone, hipEventDisableTiming));
+    grid = true;
+    return true;
+  }
+
+  // Run descriptors on the device: each run's beam arrays, parameter table,
+  // field planes, x-sweep output, partials, source, per-step factor tables
+  // and coefficients (every distinct step length's tables stay resident) and
+  // record arrays; record indices of the steps.
+  void build_run_descs(Context* c) {
     const int ns = static_cast<int>(steps.size());
     std::vector<BatchRunDesc> desc(where.size());
     dadi.resize(groups.size());
@@ -211,6 +338,10 @@ struct g4h_batch {
       }
       if (f->adi_y.size() < static_cast<size_t>(f->cells())) f->adi_y.alloc(f->ctx, f->cells());
       ensure_source(f);
+      (void)field_adi_ustar(f);
+      const int nblk = (g.G + 63) / 64;
+      const size_t need = g.runs.size() * static_cast<size_t>(nblk) * 8 + 2 * g.runs.size();
+      if (f->col_partials.size() < need) f->col_partials.alloc(f->ctx, need);
       dadi[gi].alloc(g.gctx, static_cast<size_t>(ns));
       dcs[gi].alloc(g.gctx, static_cast<size_t>(ns));
       G4_HIP(hipMemcpyAsync(dadi[gi].get(), fac.data(), sizeof(const double2*) * ns,
@@ -244,9 +375,11 @@ struct g4h_batch {
         d.adi = dadi[gi].get();
         d.cs = dcs[gi].get();
         d.field_hist = g.field_hist.get();
+        d.partials = f->col_partials.get();
         d.slice = static_cast<int>(i);
         d.nrun = static_cast<int>(g.runs.size());
         d.G = g.G;
+        d.K = 1;
         desc[static_cast<size_t>(g.runs[i])] = d;
       }
     }
@@ -257,9 +390,30 @@ struct g4h_batch {
     G4_HIP(hipMemcpyAsync(drec.get(), rec_of_step.data(), sizeof(int32_t) * rec_of_step.size(),
                           hipMemcpyHostToDevice, c->stream));
     G4_HIP(hipStreamSynchronize(c->stream));
-    G4_HIP(hipEventCreateWithFlags(&pdone, hipEventDisableTiming));
-    persistent = true;
-    return true;
+    hdesc = std::move(desc);
+  }
+
+  // Steps [k0, k1) with three or four launches per step on the batch's
+  // stream, after and before the groups' own streams.
+  void advance_grid(int k0, int k1) {
+    for (Group& g : groups) {
+      G4_HIP(hipEventRecord(g.tail, g.gctx->stream));
+      G4_HIP(hipStreamWaitEvent(ctx->stream, g.tail, 0));
+    }
+    const int nruns = static_cast<int>(druns.size());
+    for (int k = k0; k < k1; ++k) {
+      const int r = rec_of_step[static_cast<size_t>(k)];
+      const bool src = src_all[static_cast<size_t>(k)] != 0;
+      launch_batch_step_particles(ctx, prec, nharm, druns.get(), dparts.get(), nparts, k,
+                                  win_cells, r >= 0);
+      launch_batch_adi_step(ctx, prec, dblocks.get(), nblocks, k, gmax, src, r >= 0, src);
+      if (r >= 0) {
+        launch_batch_field_records(ctx, druns.get(), nruns, r);
+        ++nrec;
+      }
+    }
+    G4_HIP(hipEventRecord(pdone, ctx->stream));
+    for (Group& g : groups) G4_HIP(hipStreamWaitEvent(g.gctx->stream, pdone, 0));
   }
 
   // Steps [k0, k1) of every run in one launch on the batch's stream, after
@@ -392,7 +546,11 @@ g4h_status g4h_batch_create(g4h_context* ctx, g4h_precision precision, g4h_solve
   }
   b->build_tables();
   b->record_initial();
-  if (ctx->tuner.forced[TUNE_BATCH] == 1) b->setup_persistent(ctx);
+  const int mode = ctx->tuner.forced[TUNE_BATCH];
+  if (mode == 1)
+    b->setup_persistent(ctx);
+  else if (mode <= 0)
+    b->setup_grid(ctx);
   *out = b.release();
   context_child_added(ctx);
   G4_API_END
@@ -416,6 +574,10 @@ g4h_status g4h_batch_advance(g4h_batch* b, int nsteps, int* done) {
   const int end = std::min<int>(static_cast<int>(b->steps.size()), b->done + nsteps);
   if (b->persistent) {
     if (end > b->done) b->advance_persistent(b->done, end);
+    b->done = end;
+  }
+  if (b->grid) {
+    if (end > b->done) b->advance_grid(b->done, end);
     b->done = end;
   }
   for (; b->done < end; ++b->done) b->step(b->done);

@@ -426,6 +426,86 @@ __global__ __launch_bounds__(adi::kSegs * LPW, 4) void adi_y_seg_kernel(
   }
 }
 
+// The segmented sweeps of every run of a batch in one launch: workgroup b
+// takes block b of the descriptor list (one run's 64-line block), the run's
+// group tables and coefficient of step k.
+template <class P>
+__global__ __launch_bounds__(adi::kSegs * kLane, 4) void adi_x_seg_batch_kernel(
+    const BatchSegDesc* __restrict__ blocks, int k, int add_src) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 lds[];
+  const BatchSegDesc D = blocks[blockIdx.x];
+  const int G = D.G;
+  const double2* tab = D.adi[k];
+  for (int i = threadIdx.x; i < 5 * G; i += blockDim.x) lds[i] = tab[i];
+  __syncthreads();
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = D.rect[D.slice];
+    sp = static_cast<const T2*>(D.src) + D.slice * plane;
+  }
+  const adi::SegTables tb{lds, lds + G, lds + 2 * G, lds + 3 * G, lds + 4 * G};
+  adi::seg_sweep<P, kLane, true, false>(static_cast<const T2*>(D.u) + D.slice * plane,
+                                        static_cast<T2*>(D.ustar) + D.slice * plane, G,
+                                        (G + adi::kSegs - 1) / adi::kSegs, D.cs[k], tb, sp, rc,
+                                        D.blk, lds + 5 * G, nullptr, nullptr, nullptr);
+}
+
+template <class P>
+__global__ __launch_bounds__(adi::kSegs * kLane, 4) void adi_y_seg_batch_kernel(
+    const BatchSegDesc* __restrict__ blocks, int k, int add_src, int stats, int clear_src) {
+  using T2 = typename A2<P>::T;
+  extern __shared__ double2 lds[];
+  __shared__ double red[adi::kSegs * 8];
+  const BatchSegDesc D = blocks[blockIdx.x];
+  const int G = D.G;
+  const double2* tab = D.adi[k];
+  for (int i = threadIdx.x; i < 5 * G; i += blockDim.x) lds[i] = tab[i];
+  __syncthreads();
+  const int64_t plane = static_cast<int64_t>(G) * G;
+  int4 rc = make_int4(0, 0, 0, 0);
+  const T2* sp = nullptr;
+  if (add_src) {
+    rc = D.rect[D.slice];
+    sp = static_cast<const T2*>(D.src) + D.slice * plane;
+  }
+  const adi::SegTables tb{lds, lds + G, lds + 2 * G, lds + 3 * G, lds + 4 * G};
+  const int m = (G + adi::kSegs - 1) / adi::kSegs;
+  const T2* in = static_cast<const T2*>(D.ustar) + D.slice * plane;
+  T2* out = static_cast<T2*>(D.u) + D.slice * plane;
+  T2* clr = clear_src && sp ? const_cast<T2*>(sp) : nullptr;
+  if (stats) {
+    double* part = D.partials + (static_cast<int64_t>(D.slice) * D.nblk + D.blk) * 8;
+    double* centre = D.partials + static_cast<int64_t>(D.nrun) * D.nblk * 8 + 2 * D.slice;
+    adi::seg_sweep<P, kLane, false, true>(in, out, G, m, D.cs[k], tb, sp, rc, D.blk,
+                                          lds + 5 * G, red, part, centre, clr);
+  } else {
+    adi::seg_sweep<P, kLane, false, false>(in, out, G, m, D.cs[k], tb, sp, rc, D.blk,
+                                           lds + 5 * G, nullptr, nullptr, nullptr, clr);
+  }
+}
+
+// Field record r of every run from its block partials, summed in block order
+// (as reduce_partials_kernel sums them for a single field).
+__global__ void batch_field_record_kernel(const BatchRunDesc* __restrict__ runs, int nruns,
+                                          int r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nruns) return;
+  const BatchRunDesc R = runs[i];
+  const int nblk = (R.G + kLane - 1) / kLane;
+  double* o = R.field_hist + (static_cast<int64_t>(r) * R.nrun + R.slice) * G4H_FIELD_RAW;
+  for (int q = 0; q < 7; ++q) {
+    double v = 0.0;
+    for (int t = 0; t < nblk; ++t) v += R.partials[(static_cast<int64_t>(R.slice) * nblk + t) * 8 + q];
+    o[q] = v;
+  }
+  const double* c = R.partials + static_cast<int64_t>(R.nrun) * nblk * 8 + 2 * R.slice;
+  o[7] = c[0];
+  o[8] = c[1];
+}
+
 // The x sweep's output of the wave-per-block variants (the y sweep's input),
 // one plane per ring slot in the field's storage type.
 void* adi_ustar(Field* f) {
@@ -544,6 +624,41 @@ void adi_step(Field* f, bool source_on, bool stats) {
 }
 
 }  // namespace
+
+void* field_adi_ustar(Field* f) { return adi_ustar(f); }
+
+void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc* d_blocks,
+                           int nblocks, int k, int gmax, bool add_src, bool stats,
+                           bool clear_src) {
+  if (nblocks <= 0) return;
+  require(gmax <= adi::kSegs * adi::kSegMax, "segmented sweeps: ngrid above 256");
+  const size_t lds = sizeof(double2) * (5 * static_cast<size_t>(gmax) + adi::kSegs * kLane);
+  const int src = add_src ? 1 : 0, st = stats ? 1 : 0, clr = clear_src ? 1 : 0;
+  ctx->launch(G4H_K_ADI_X, [&] {
+    if (prec == G4H_FP64)
+      hipLaunchKernelGGL(adi_x_seg_batch_kernel<double>, dim3(nblocks), dim3(adi::kSegs * kLane),
+                         lds, ctx->stream, d_blocks, k, src);
+    else
+      hipLaunchKernelGGL(adi_x_seg_batch_kernel<float>, dim3(nblocks), dim3(adi::kSegs * kLane),
+                         lds, ctx->stream, d_blocks, k, src);
+  });
+  ctx->launch(G4H_K_ADI_Y, [&] {
+    if (prec == G4H_FP64)
+      hipLaunchKernelGGL(adi_y_seg_batch_kernel<double>, dim3(nblocks), dim3(adi::kSegs * kLane),
+                         lds, ctx->stream, d_blocks, k, src, st, clr);
+    else
+      hipLaunchKernelGGL(adi_y_seg_batch_kernel<float>, dim3(nblocks), dim3(adi::kSegs * kLane),
+                         lds, ctx->stream, d_blocks, k, src, st, clr);
+  });
+}
+
+void launch_batch_field_records(Context* ctx, const BatchRunDesc* d_runs, int nruns, int r) {
+  if (nruns <= 0) return;
+  ctx->launch(G4H_K_OTHER, [&] {
+    hipLaunchKernelGGL(batch_field_record_kernel, dim3((nruns + 127) / 128), dim3(128), 0,
+                       ctx->stream, d_runs, nruns, r);
+  });
+}
 
 // Thomas factors of FieldSolverADICUDA::init (FieldSolverADICUDA.cpp:821-866).
 void adi_prepare(Field* f, double delz) {

@@ -108,8 +108,8 @@ enum TuneFamily : int {
   TUNE_ADI = 0,    // 0: workgroup per plane (LDS-staged), 1: one wave per 64 lines
   TUNE_SPLIT = 1,  // workgroups per slice of the particle pass: not tuned, chosen
                    // from the slice count; pinning sets it (>= 1)
-  TUNE_BATCH = 2,  // ADI batches: 0 one launch per kernel and step (default), 1 one
-                   // launch for all steps
+  TUNE_BATCH = 2,  // ADI batches: 0 one launch per kernel and step for every run
+                   // (default), 1 one launch for all steps, 2 per-grid launches
   TUNE_COUNT
 };
 

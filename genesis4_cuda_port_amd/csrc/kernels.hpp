@@ -134,11 +134,49 @@ struct BatchRunDesc {
   const double2* const* adi;   // per step: Thomas factors c, cbet, cwet [3G]
   const double* cs;            // per step: the sweeps' i*rtmp coefficient
   double* field_hist;          // group field records [record][nrun][G4H_FIELD_RAW]
+  double* partials;            // group field-sum partials of the segmented y sweep
+  // the run's own split scratch when its particle pass takes K > 1 parts
+  double2* swin;               // [K][window cells]
+  double* sred;                // [K][sums]
+  int4* sfb;                   // [K]
+  int32_t* scnt;               // arrival counter (zero between launches)
   int slice;                   // the run's slice (= plane) in its group
   int nrun;                    // runs in the group
   int G;
-  int pad_;
+  int K;                       // parts of the run's particle pass (per-step launches)
 };
+
+// One 64-line block of one run's plane for the segmented sweeps of a whole
+// batch in one launch (launch_batch_adi_step).
+struct BatchSegDesc {
+  void* u;                     // group field planes (double2* / float2*)
+  void* ustar;                 // group x-sweep output planes
+  const void* src;             // group source planes
+  const int4* rect;            // group source rectangles
+  double* partials;            // group partials [nrun][nblk][8], then 2 centre values per run
+  const double2* const* adi;   // per step: the group's c, cbet, cwet, beta, delta [5G]
+  const double* cs;            // per step: the sweeps' i*rtmp coefficient
+  int G, nblk, nrun, slice, blk, pad_;
+};
+
+// One step k of every run of a batch, each kernel in one launch: the fused
+// particle passes (one workgroup per run, the run's parameter entry of the
+// step), the two segmented sweeps of every run's plane (one workgroup per
+// 64-line block, the run's group tables of the step; G <= 256), and on
+// output steps every run's field record from its sweep partials (record r).
+// d_parts: (run, part) of each workgroup, every run's K parts in order.  A
+// split run's parts deposit their out-of-window particles with atomics onto
+// its source plane, which launch_batch_adi_step clears behind its y sweep
+// (clear_src) so that each step's deposition starts from zero.
+void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
+                                 const BatchRunDesc* d_runs, const int2* d_parts, int nparts,
+                                 int k, int win_cells, bool sums);
+void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc* d_blocks,
+                           int nblocks, int k, int gmax, bool add_src, bool stats,
+                           bool clear_src);
+// split scratch of a batch run: window cells, sums per part
+size_t batch_split_red_stride();
+void launch_batch_field_records(Context* ctx, const BatchRunDesc* d_runs, int nruns, int r);
 
 // Steps [k0, k1) of every run of a batch of steady-state ADI runs in ONE
 // launch: one workgroup per run loops over the steps (particle pass, both
@@ -182,6 +220,7 @@ bool make_dep(DepParams& d, Field* f, Beam* b, const g4h_step& st, double delz);
 
 // ---- field solvers -------------------------------------------------------
 void field_prepare_solver(Field* f, double delz);  // (re)build coefficient tables
+void* field_adi_ustar(Field* f);  // the ADI x sweep's output planes (allocated on first use)
 // One field step with the source (if source_on).  When stats != 0 the last
 // pass also writes per-ring-slot field sums into f->plane_stats.
 void field_solve(Field* f, bool source_on, bool stats);

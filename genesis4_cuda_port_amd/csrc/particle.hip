@@ -765,6 +765,7 @@ struct SplitScratch {
   double* red;   // [nslice][K][kRedStride]
   int4* fb;      // [nslice][K]
   int32_t* cnt;  // [nslice]
+  int own;       // nonzero: the arrays are this slice's own ([K][..], one counter)
 };
 
 template <class P, int NF, bool DEP, int NH>
@@ -1168,7 +1169,8 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     // this part's window, fallback box and sums to scratch; the last part
     // to arrive goes on to combine all K of them in part order
     __syncthreads();
-    const int64_t part = static_cast<int64_t>(s) * K + kp;
+    const int64_t ss = sp.own ? 0 : s;  // the slice's slot in the scratch arrays
+    const int64_t part = ss * K + kp;
     if constexpr (DEP) if (dep_on) {
       double2* w = sp.win + part * prm.dep.window_cap;
       for (int k = tid; k < ww * wh; k += NT) w[k] = win[k];
@@ -1191,21 +1193,21 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     __threadfence();
     __syncthreads();
     __shared__ int s_last;
-    if (tid == 0) s_last = atomicAdd(&sp.cnt[s], 1) == K - 1;
+    if (tid == 0) s_last = atomicAdd(&sp.cnt[ss], 1) == K - 1;
     __syncthreads();
     if (!s_last) return;
     __threadfence();
     if (tid == 0) {
       if constexpr (DEP) if (dep_on) {
         for (int q = 0; q < K; ++q) {
-          const int4 f = sp.fb[static_cast<int64_t>(s) * K + q];
+          const int4 f = sp.fb[ss * K + q];
           s_fb[0] = min(s_fb[0], f.x);
           s_fb[1] = min(s_fb[1], f.y);
           s_fb[2] = max(s_fb[2], f.z);
           s_fb[3] = max(s_fb[3], f.w);
         }
       }
-      sp.cnt[s] = 0;
+      sp.cnt[ss] = 0;
     }
     __syncthreads();
   }
@@ -1215,7 +1217,7 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     // write the window with plain stores; its non-zero cells give the
     // footprint for the next step's window
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
-    const double2* parts = sp.win + static_cast<int64_t>(s) * K * prm.dep.window_cap;
+    const double2* parts = sp.win + (sp.own ? 0 : static_cast<int64_t>(s)) * K * prm.dep.window_cap;
     for (int k = tid; k < ww * wh; k += NT) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
       double2 v = win[k];
@@ -1269,7 +1271,7 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
 
   if constexpr (NH > 0) if (diag_on) {
     __syncthreads();
-    const double* parts = sp.red + static_cast<int64_t>(s) * K * kRedStride;
+    const double* parts = sp.red + (sp.own ? 0 : static_cast<int64_t>(s)) * K * kRedStride;
     double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
     // raw layout: 12 sums (G4H_B_G1..G4H_B_YPY), the count, 10 min/max, then
     // (cos, sin) sums per harmonic
@@ -1572,6 +1574,25 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
   }
 }
 
+// One step of every run: workgroup r runs the fused particle pass of run r
+// (unsplit, its parameter entry of step k; deposition and sums as the step's
+// stages ask).
+// Workgroups of kThreads as the grid launch's, and the beam sums compiled
+// in only for output steps (NH > 0): a register count that keeps several
+// workgroups per CU.
+template <class P, int NH>
+__global__ __launch_bounds__(kThreads) void batch_step_particle_kernel(
+    const BatchRunDesc* __restrict__ runs, const int2* __restrict__ parts, int k) {
+  extern __shared__ double2 lds[];
+  const int2 pr = parts[blockIdx.x];  // (run, part)
+  const BatchRunDesc& R = runs[pr.x];
+  const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
+  BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
+               static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
+  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1};
+  particle_slice<P, 1, true, NH, kThreads, true>(b, R.off, prm, sp, R.slice, R.K, pr.y, lds);
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1696,6 +1717,37 @@ void launch_batch_persistent(Context* ctx, g4h_precision prec, int nharm,
                          k0, k1, win_cells);
     });
   });
+}
+
+size_t batch_split_red_stride() { return kRedStride; }
+
+void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
+                                 const BatchRunDesc* d_runs, const int2* d_parts, int nparts,
+                                 int k, int win_cells, bool sums) {
+  if (nparts <= 0) return;
+  require(nharm >= 1 && nharm <= kMaxHarm, "nharm must be in [1, 8]");
+  const size_t lds = sizeof(double2) * static_cast<size_t>(win_cells);
+  auto go = [&](auto kern) {
+    ctx->launch(G4H_K_PARTICLE, [&] {
+      hipLaunchKernelGGL(kern, dim3(nparts), dim3(kThreads), lds, ctx->stream, d_runs, d_parts,
+                         k);
+    });
+  };
+  if (prec == G4H_FP64) {
+    if (!sums)
+      go(batch_step_particle_kernel<double, 0>);
+    else if (nharm == 1)
+      go(batch_step_particle_kernel<double, 1>);
+    else
+      go(batch_step_particle_kernel<double, 8>);
+  } else {
+    if (!sums)
+      go(batch_step_particle_kernel<float, 0>);
+    else if (nharm == 1)
+      go(batch_step_particle_kernel<float, 1>);
+    else
+      go(batch_step_particle_kernel<float, 8>);
+  }
 }
 
 // ---- beam sums without allocation ----------------------------------------

@@ -124,7 +124,7 @@ g4h_status g4h_context_set_variant(g4h_context* ctx, int family, int variant) {
   require(family >= 0 && family < TUNE_COUNT, "unknown kernel family");
   require(variant >= -1, "variant must be >= -1 (-1: autotune)");
   require(family != TUNE_SPLIT || variant != 0, "workgroups per slice must be >= 1");
-  require(family != TUNE_BATCH || variant <= 1, "batch variant must be -1, 0 or 1");
+  require(family != TUNE_BATCH || variant <= 2, "batch variant must be -1 .. 2");
   require(family != TUNE_ADI || variant <= 3, "ADI variant must be -1 .. 3");
   ctx->tuner.forced[family] = variant;
   G4_API_END

@@ -72,10 +72,12 @@ g4h_status g4h_context_memory(const g4h_context* ctx, uint64_t* bytes_in_use);
                             slices still fill the GPU; results agree to
                             round-off, not bit for bit) */
 #define G4H_TUNE_BATCH 2 /* batches of steady-state ADI runs: 0 (the default)
-                            one launch per kernel and step on per-grid
-                            streams, 1 every step of every run in one launch
-                            (one workgroup per run looping over the steps;
-                            grids <= 512); results agree to round-off */
+                            each step of every run as one launch per kernel
+                            (grids <= 256, else as 2), 1 every step of every
+                            run in one launch (one workgroup per run looping
+                            over the steps; grids <= 512), 2 one launch per
+                            kernel, step and grid on per-grid streams;
+                            results agree to round-off */
 /* Pin `family` to `variant` on this context (-1: autotune, the default). */
 g4h_status g4h_context_set_variant(g4h_context* ctx, int family, int variant);
 /* Decided variant of `family` for a shape ({ngrid, nslice, precision} for

@@ -7,9 +7,10 @@ and one run is checked against the FP64 oracle.
 
 Bars: FP64 1e-12 relative against the lone run and 1e-10 against the
 oracle on slice power and bunching; MIXED 1e-6 against the lone MIXED run.
-ADI batches can also run as one launch over all steps (G4H_TUNE_BATCH = 1);
-that batch must agree with the default one launch per kernel and step to
-the same bars."""
+ADI batches run by default as one launch per kernel and step for all runs
+(G4H_TUNE_BATCH = 0), and can also run as one launch over all steps (1) or
+as per-grid launches on per-grid streams (2); the three must agree to the
+same bars (and on particle counts exactly)."""
 from __future__ import annotations
 
 import numpy as np
@@ -117,14 +118,14 @@ def test_batch_runs_match_lone_runs(gpu_ctx, prec, solver):
 
 
 @pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
-def test_batch_one_launch_matches_per_step_launches(prec):
-    print(f"case batch one launch vs per step prec={prec}", flush=True)
+def test_batch_launch_modes_agree(prec):
+    print(f"case batch launch modes prec={prec}", flush=True)
     runs = _runs()
     runs.append(dict(runs[0], npart=4096, ngrid=101, seed=16, seed_power=1e5))
     runs.append(dict(runs[1], npart=2048, ngrid=255, seed=17))
     steps = _steps()
     out = {}
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         with api.Context(0) as ctx:
             ctx.set_variant(TUNE_BATCH, variant)
             batch = api.Batch(ctx, prec, api.ADI, runs, steps, LAMBDA0)
@@ -135,11 +136,13 @@ def test_batch_one_launch_matches_per_step_launches(prec):
             out[variant] = [batch.download(i) for i in range(len(runs))]
             batch.close()
     tol = 1e-12 if prec == api.FP64 else 1e-6
-    for i, r in enumerate(runs):
-        (b0, f0), (b1, f1) = out[0][i], out[1][i]
-        P0, B0 = _power_bunching(b0[:, None, :], f0[:, None, :], r["ngrid"], r["current"])
-        P1, B1 = _power_bunching(b1[:, None, :], f1[:, None, :], r["ngrid"], r["current"])
-        ep, eb = relerr(P1, P0), relerr(B1, B0)
-        print(f"  run {i} (ngrid {r['ngrid']}): power {ep:.3e} bunching {eb:.3e}", flush=True)
-        assert ep < tol and eb < tol, i
-        assert np.array_equal(b0[:, 12], b1[:, 12])  # particle counts
+    for v in (1, 2):
+        for i, r in enumerate(runs):
+            (b0, f0), (b1, f1) = out[0][i], out[v][i]
+            P0, B0 = _power_bunching(b0[:, None, :], f0[:, None, :], r["ngrid"], r["current"])
+            P1, B1 = _power_bunching(b1[:, None, :], f1[:, None, :], r["ngrid"], r["current"])
+            ep, eb = relerr(P1, P0), relerr(B1, B0)
+            print(f"  variant {v} run {i} (ngrid {r['ngrid']}): power {ep:.3e} bunching {eb:.3e}",
+                  flush=True)
+            assert ep < tol and eb < tol, (v, i)
+            assert np.array_equal(b0[:, 12], b1[:, 12])  # particle counts
