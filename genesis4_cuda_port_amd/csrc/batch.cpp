@@ -93,8 +93,7 @@ struct g4h_batch {
   std::vector<BatchRunDesc> hdesc;  // host copy of druns
   DevBuf<int2> dparts;              // (run, part) of each particle workgroup
   int nparts = 0;
-  DevBuf<double2> swin;             // split scratch of every run
-  DevBuf<double> sred;
+  DevBuf<double> sred;              // split scratch of every run
   DevBuf<int4> sfb;
   DevBuf<int32_t> scnt;
   DevBuf<BatchRunDesc> druns;
@@ -106,7 +105,6 @@ struct g4h_batch {
   ~g4h_batch() {
     dblocks.release();
     dparts.release();
-    swin.release();
     sred.release();
     sfb.release();
     scnt.release();
@@ -248,7 +246,8 @@ struct g4h_batch {
     build_run_descs(c);
     // Particle workgroups: a run of n particles takes K = ceil(n / 2048)
     // parts (at most 16), eight iterations of the 256-thread loop each, so
-    // that a batch of runs of a few thousand particles fills the GPU.
+    // that a batch of runs of a few thousand particles fills the GPU.  Each
+    // part adds its deposition window to the source plane with atomics.
     std::vector<int2> parts;
     std::vector<int> kof(hdesc.size());
     size_t ktot = 0;
@@ -260,17 +259,21 @@ struct g4h_batch {
       for (int q = 0; q < kof[r]; ++q) parts.push_back(make_int2(static_cast<int>(r), q));
       ktot += static_cast<size_t>(kof[r]);
     }
+    // split scratch: each part adds its window to the source plane with
+    // atomics (no window copies); fb holds each run's two boxes, empty
     const size_t red = batch_split_red_stride();
-    swin.alloc(c, ktot * static_cast<size_t>(win_cells));
     sred.alloc(c, ktot * red);
     sfb.alloc(c, ktot);
     scnt.alloc(c, hdesc.size());
     G4_HIP(hipMemsetAsync(scnt.get(), 0, sizeof(int32_t) * hdesc.size(), c->stream));
+    const std::vector<int4> empty(ktot, make_int4(0x7fffffff, 0x7fffffff, -1, -1));
+    G4_HIP(hipMemcpyAsync(sfb.get(), empty.data(), sizeof(int4) * ktot, hipMemcpyHostToDevice,
+                          c->stream));
     size_t at = 0;
     for (size_t r = 0; r < hdesc.size(); ++r) {
       BatchRunDesc& d = hdesc[r];
       d.K = kof[r];
-      d.swin = swin.get() + at * static_cast<size_t>(win_cells);
+      d.swin = nullptr;
       d.sred = sred.get() + at * red;
       d.sfb = sfb.get() + at;
       d.scnt = scnt.get() + r;

@@ -766,6 +766,11 @@ struct SplitScratch {
   int4* fb;      // [nslice][K]
   int32_t* cnt;  // [nslice]
   int own;       // nonzero: the arrays are this slice's own ([K][..], one counter)
+  // nonzero: each part adds its window to the (cleared) source plane with
+  // atomics and folds its boxes into fb[0] (non-zero cells) and fb[1] (the
+  // written region), both empty between launches; the last part only
+  // publishes them (no K-window combine; win is unused)
+  int atomic_win;
 };
 
 template <class P, int NF, bool DEP, int NH>
@@ -1172,9 +1177,52 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     const int64_t ss = sp.own ? 0 : s;  // the slice's slot in the scratch arrays
     const int64_t part = ss * K + kp;
     if constexpr (DEP) if (dep_on) {
-      double2* w = sp.win + part * prm.dep.window_cap;
-      for (int k = tid; k < ww * wh; k += NT) w[k] = win[k];
-      if (tid == 0) sp.fb[part] = make_int4(s_fb[0], s_fb[1], s_fb[2], s_fb[3]);
+      if (sp.atomic_win) {
+        int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+        for (int k = tid; k < ww * wh; k += NT) {
+          const double2 v = win[k];
+          if (v.x != 0.0 || v.y != 0.0) {
+            const int cy = wy0 + k / ww, cx = wx0 + k % ww;
+            P* a = reinterpret_cast<P*>(&src[cx + static_cast<int64_t>(cy) * ng]);
+            glob_add(a, static_cast<P>(v.x));
+            glob_add(a + 1, static_cast<P>(v.y));
+            mnx = min(mnx, cx);
+            mny = min(mny, cy);
+            mxx = max(mxx, cx);
+            mxy = max(mxy, cy);
+          }
+        }
+        atomicMin(&s_bb[0], mnx);
+        atomicMin(&s_bb[1], mny);
+        atomicMax(&s_bb[2], mxx);
+        atomicMax(&s_bb[3], mxy);
+        __syncthreads();
+        if (tid == 0) {
+          int4 t = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
+          int4 r = ww > 0 && wh > 0 ? make_int4(wx0, wy0, wx0 + ww, wy0 + wh)
+                                    : make_int4(0x7fffffff, 0x7fffffff, -1, -1);
+          if (s_fb[0] <= s_fb[2]) {
+            r = make_int4(min(r.x, s_fb[0]), min(r.y, s_fb[1]), max(r.z, s_fb[2] + 1),
+                          max(r.w, s_fb[3] + 1));
+            t = make_int4(min(t.x, s_fb[0]), min(t.y, s_fb[1]), max(t.z, s_fb[2]),
+                          max(t.w, s_fb[3]));
+          }
+          int4* bt = &sp.fb[ss * K];
+          int4* br = &sp.fb[ss * K + 1];
+          atomicMin(&bt->x, t.x);
+          atomicMin(&bt->y, t.y);
+          atomicMax(&bt->z, t.z);
+          atomicMax(&bt->w, t.w);
+          atomicMin(&br->x, r.x);
+          atomicMin(&br->y, r.y);
+          atomicMax(&br->z, r.z);
+          atomicMax(&br->w, r.w);
+        }
+      } else {
+        double2* w = sp.win + part * prm.dep.window_cap;
+        for (int k = tid; k < ww * wh; k += NT) w[k] = win[k];
+        if (tid == 0) sp.fb[part] = make_int4(s_fb[0], s_fb[1], s_fb[2], s_fb[3]);
+      }
     }
     if constexpr (NH > 0) if (diag_on) {
       double* r = sp.red + part * kRedStride;
@@ -1199,12 +1247,19 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     __threadfence();
     if (tid == 0) {
       if constexpr (DEP) if (dep_on) {
-        for (int q = 0; q < K; ++q) {
-          const int4 f = sp.fb[ss * K + q];
-          s_fb[0] = min(s_fb[0], f.x);
-          s_fb[1] = min(s_fb[1], f.y);
-          s_fb[2] = max(s_fb[2], f.z);
-          s_fb[3] = max(s_fb[3], f.w);
+        if (sp.atomic_win) {
+          const int4 t = sp.fb[ss * K], r = sp.fb[ss * K + 1];
+          prm.dep.rect[s] = r.x > r.z ? make_int4(0, 0, 0, 0) : r;
+          prm.dep.bbox[s] = t;
+          sp.fb[ss * K] = sp.fb[ss * K + 1] = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
+        } else {
+          for (int q = 0; q < K; ++q) {
+            const int4 f = sp.fb[ss * K + q];
+            s_fb[0] = min(s_fb[0], f.x);
+            s_fb[1] = min(s_fb[1], f.y);
+            s_fb[2] = max(s_fb[2], f.z);
+            s_fb[3] = max(s_fb[3], f.w);
+          }
         }
       }
       sp.cnt[ss] = 0;
@@ -1212,7 +1267,7 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     __syncthreads();
   }
 
-  if constexpr (DEP) if (dep_on) {
+  if constexpr (DEP) if (dep_on && !(K > 1 && sp.atomic_win)) {
     __syncthreads();
     // write the window with plain stores; its non-zero cells give the
     // footprint for the next step's window
@@ -1589,7 +1644,7 @@ __global__ __launch_bounds__(kThreads) void batch_step_particle_kernel(
   const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
   BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
                static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
-  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1};
+  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1, 1};
   particle_slice<P, 1, true, NH, kThreads, true>(b, R.off, prm, sp, R.slice, R.K, pr.y, lds);
 }
 
