@@ -6,21 +6,25 @@ times, alternating between two ensemble members (beam and seed field of
 different seeds), with eager work and synchronisations in between.  Every
 replay must give the records of the same member run eagerly in a fresh
 context: FP64 within 1e-12 relative (the deposition sums may add in another
-order), MIXED within 1e-5.  Replaying without re-initialising the field is
-refused."""
+order), MIXED within 1e-5.  The first replay is also checked against the
+FP64 oracle run of its member (slice power and bunching: FP64 1e-10, MIXED
+1e-3).  Replaying without re-initialising the field is refused."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
+import _oracle
+from _oracle import OField
 import cases
-from cases import LAMBDA0, quad_step
+from cases import GAMMA0, LAMBDA0, quad_step
 from genesis4_cuda_port_amd import api
 from genesis4_cuda_port_amd._abi import G4Error
 
 pytestmark = pytest.mark.gpu
 
-from test_gpu_parity import gpu_beam, gpu_field, relerr  # noqa: E402
+from test_gpu_parity import (FP64_TOL, MIXED_TOL, _power_bunching, gpu_beam,  # noqa: E402
+                             gpu_field, relerr)
 
 
 def _eager(prec, kind, c):
@@ -66,6 +70,18 @@ def test_graph_replay_matches_eager(prec, kind):
             eb, ef = relerr(braw, ref[k][0]), relerr(fraw, ref[k][1])
             print(f"  replay {rep} (member {k}): beam {eb:.3e} field {ef:.3e}", flush=True)
             assert eb < tol and ef < tol
+            if rep == 0:
+                of = OField(m.nslice, m.ngrid, m.gridmax, LAMBDA0, m.slicelength, 1, m.seed_re,
+                            m.seed_im)
+                _, obraw, ofraw = _oracle.run(m.beam, of, m.steps, gamma0=GAMMA0,
+                                              lambda0=LAMBDA0, solver=kind, sample=m.sample,
+                                              timerun=m.timerun, nharm=2)
+                P, B = _power_bunching(braw, fraw, m, f)
+                oP, oB = _power_bunching(obraw, ofraw, m, f)
+                ep, eb2 = relerr(P[1:], oP[1:]), relerr(B, oB)
+                print(f"  replay vs oracle: power {ep:.3e} bunching {eb2:.3e}", flush=True)
+                otol = FP64_TOL if prec == api.FP64 else MIXED_TOL
+                assert ep < otol and eb2 < otol
             # eager work on the same stream between replays
             b.diag_raw(2)
             ctx.synchronize()
