@@ -451,9 +451,10 @@ __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
   constexpr int E = sizeof(T2);
   constexpr int M = kSegMax;
   constexpr int NW = kSegs * LPW / kLane;  // waves of the workgroup
-  // edge and source loads ahead (fewer with the field sums, which hold
-  // their own registers: 128 per lane fit four waves per SIMD)
-  constexpr int CH = (sizeof(P) == 8 ? 2 : 4) / (STATS ? 2 : 1);
+  // edge and source loads ahead (fewer in the y sweep, whose field sums and
+  // source clearing hold their own registers: 128 per lane fit four waves
+  // per SIMD)
+  constexpr int CH = (sizeof(P) == 8 ? 2 : 4) / (XDIR ? 1 : 2);
   constexpr int AIN = XDIR ? 0 : kNT;  // the y sweep reads its lines once
   const int t = threadIdx.x & (kLane - 1);
   const int l = threadIdx.x % LPW, j = threadIdx.x / LPW;
@@ -519,17 +520,6 @@ __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
       }
     }
   }
-  if (sclear) {
-    const Rsrc rw = plane_rsrc(sclear, G, E);
-    T2 z;
-    z.x = 0;
-    z.y = 0;
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-      const int k = s + c;
-      bstore<0>(rw, off(sline && c < n && k >= klo && k < khi, L, k), z);
-    }
-  }
   // 2. the forward value entering the segment, from the end values of the
   //    segments before it
   sA[j * LPW + l] = a;
@@ -575,14 +565,17 @@ __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
       const T2 o = from_d<P>(uu);
       bstore<AIN>(ro, off(act, L, k), o);
       if constexpr (STATS) {
+        // the lane's column L is fixed: its x moments are taken from the sum
+        // of |u|^2 after the loop (acc[1], acc[2] hold the y ones meanwhile)
         const double re = o.x, im = o.y;
         const double wei = re * re + im * im;
-        const double dx = L + shift, dy = k + shift;  // column L, row k
-        const double v[7] = {wei, dx * wei, dx * dx * wei, dy * wei, dy * dy * wei, re, im};
+        const double dy = k + shift;  // row k
+        const double v[5] = {wei, dy * wei, dy * dy * wei, re, im};
+        constexpr int slot[5] = {0, 3, 4, 5, 6};
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
-          const double nq = sums.acc[q] + v[q];
-          sums.acc[q] = act ? nq : sums.acc[q];
+        for (int q = 0; q < 5; ++q) {
+          const double nq = sums.acc[slot[q]] + v[q];
+          sums.acc[slot[q]] = act ? nq : sums.acc[slot[q]];
         }
         if (k == crow && L == ccol) {
           centre[0] = re;
@@ -591,7 +584,21 @@ __device__ __forceinline__ void seg_sweep(const typename A2<P>::T* uin,
       }
     }
   }
+  if (sclear) {  // after the stores, when the segment's registers are free
+    const Rsrc rw = plane_rsrc(sclear, G, E);
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      const int k = s + c;
+      bstore<0>(rw, off(sline && c < n && k >= klo && k < khi, L, k), z);
+    }
+  }
   if constexpr (STATS) {
+    const double dx = L + shift;  // column L
+    sums.acc[1] = dx * sums.acc[0];
+    sums.acc[2] = dx * dx * sums.acc[0];
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
       const double v = wave_sum64(sums.acc[q]);

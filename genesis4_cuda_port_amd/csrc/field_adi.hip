@@ -453,9 +453,9 @@ __global__ __launch_bounds__(adi::kSegs * kLane, 4) void adi_x_seg_batch_kernel(
                                         D.blk, lds + 5 * G, nullptr, nullptr, nullptr);
 }
 
-template <class P>
+template <class P, bool STATS>
 __global__ __launch_bounds__(adi::kSegs * kLane, 4) void adi_y_seg_batch_kernel(
-    const BatchSegDesc* __restrict__ blocks, int k, int add_src, int stats, int clear_src) {
+    const BatchSegDesc* __restrict__ blocks, int k, int add_src, int clear_src) {
   using T2 = typename A2<P>::T;
   extern __shared__ double2 lds[];
   __shared__ double red[adi::kSegs * 8];
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(adi::kSegs * kLane, 4) void adi_y_seg_batch_kernel(
   const T2* in = static_cast<const T2*>(D.ustar) + D.slice * plane;
   T2* out = static_cast<T2*>(D.u) + D.slice * plane;
   T2* clr = clear_src && sp ? const_cast<T2*>(sp) : nullptr;
-  if (stats) {
+  if constexpr (STATS) {
     double* part = D.partials + (static_cast<int64_t>(D.slice) * D.nblk + D.blk) * 8;
     double* centre = D.partials + static_cast<int64_t>(D.nrun) * D.nblk * 8 + 2 * D.slice;
     adi::seg_sweep<P, kLane, false, true>(in, out, G, m, D.cs[k], tb, sp, rc, D.blk,
@@ -633,7 +633,7 @@ void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc*
   if (nblocks <= 0) return;
   require(gmax <= adi::kSegs * adi::kSegMax, "segmented sweeps: ngrid above 256");
   const size_t lds = sizeof(double2) * (5 * static_cast<size_t>(gmax) + adi::kSegs * kLane);
-  const int src = add_src ? 1 : 0, st = stats ? 1 : 0, clr = clear_src ? 1 : 0;
+  const int src = add_src ? 1 : 0, clr = clear_src ? 1 : 0;
   ctx->launch(G4H_K_ADI_X, [&] {
     if (prec == G4H_FP64)
       hipLaunchKernelGGL(adi_x_seg_batch_kernel<double>, dim3(nblocks), dim3(adi::kSegs * kLane),
@@ -642,14 +642,23 @@ void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc*
       hipLaunchKernelGGL(adi_x_seg_batch_kernel<float>, dim3(nblocks), dim3(adi::kSegs * kLane),
                          lds, ctx->stream, d_blocks, k, src);
   });
-  ctx->launch(G4H_K_ADI_Y, [&] {
-    if (prec == G4H_FP64)
-      hipLaunchKernelGGL(adi_y_seg_batch_kernel<double>, dim3(nblocks), dim3(adi::kSegs * kLane),
-                         lds, ctx->stream, d_blocks, k, src, st, clr);
+  auto y = [&](auto kern) {
+    ctx->launch(G4H_K_ADI_Y, [&] {
+      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(adi::kSegs * kLane), lds, ctx->stream,
+                         d_blocks, k, src, clr);
+    });
+  };
+  if (prec == G4H_FP64) {
+    if (stats)
+      y(adi_y_seg_batch_kernel<double, true>);
     else
-      hipLaunchKernelGGL(adi_y_seg_batch_kernel<float>, dim3(nblocks), dim3(adi::kSegs * kLane),
-                         lds, ctx->stream, d_blocks, k, src, st, clr);
-  });
+      y(adi_y_seg_batch_kernel<double, false>);
+  } else {
+    if (stats)
+      y(adi_y_seg_batch_kernel<float, true>);
+    else
+      y(adi_y_seg_batch_kernel<float, false>);
+  }
 }
 
 void launch_batch_field_records(Context* ctx, const BatchRunDesc* d_runs, int nruns, int r) {
