@@ -217,8 +217,12 @@ TransParams make_trans(const g4h_step& st, double gamma0, double delz, bool last
   {
     // K = str/gammaz with gammaz >= gammaz(gamma0)/2 for gamma >= gamma0/2
     const double gz_min = 0.5 * std::sqrt(std::max(t.g2c, 1.0));
-    t.xpoly = t.xmode != 0 && std::fabs(t.xstr) / gz_min * delz * delz <= 0.5 ? 1 : 0;
-    t.ypoly = t.ymode != 0 && std::fabs(t.ystr) / gz_min * delz * delz <= 0.5 ? 1 : 0;
+    auto series = [&](int mode, double str) {
+      const double w = std::fabs(str) / gz_min * delz * delz;
+      return mode == 0 || w > 0.5 ? 0 : (w <= 0.011 ? 2 : 1);
+    };
+    t.xpoly = series(t.xmode, t.xstr);
+    t.ypoly = series(t.ymode, t.ystr);
   }
   t.g2chic = gamma0 * gamma0 - 1.0;
   if (last) {

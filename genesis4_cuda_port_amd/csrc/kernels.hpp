@@ -59,8 +59,9 @@ struct TransParams {
   double g2chic;   // gamma0^2 - 1          (mixed chicane gammaz)
   int xmode, ymode;  // 0 drift, 1 focusing, 2 defocusing
   int chicane, corrector;
-  // FP32 may use the power series of the plane maps in K dz^2 (|K dz^2|
-  // <= 1/2 for every particle with gamma >= gamma0/2)
+  // plane maps from their power series in w = K dz^2, bounded over every
+  // particle with gamma >= gamma0/2: 0 no (trigonometric form), 1 |w| <= 1/2
+  // (long series), 2 |w| <= 0.011 (short series)
   int xpoly, ypoly;
 };
 

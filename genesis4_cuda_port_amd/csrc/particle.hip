@@ -121,14 +121,16 @@ __device__ __forceinline__ void plane_step(int mode, P str, P off, P delz, P& co
 }
 
 // MIXED overload: fast reciprocal / square root and hardware sin/cos (the
-// focusing phase advance per half step is ~1e-2 rad).  With `poly` the map
-// is taken from its power series in w = K dz^2 (K = str/gammaz, either
+// focusing phase advance per half step is ~1e-2 rad).  With `poly` > 0 the
+// map is taken from its power series in w = K dz^2 (K = str/gammaz, either
 // sign: cos(sqrt(w)) and sin(sqrt(w))/sqrt(w) are entire in w and cover
-// focusing and defocusing alike), to w^4: for |w| <= 1/2 the remainders are
-// below 1e-8, under FP32 rounding, and no transcendental is evaluated.
+// focusing and defocusing alike) and no transcendental is evaluated: to w^4
+// for |w| <= 1/2 (poly 1), to w^2 for |w| <= 0.011 (poly 2).  The series
+// alternate with decreasing terms, so the remainder is below the first
+// omitted term: w^5/10! < 1e-8 and w^3/6! < 2e-9, under FP32 rounding.
 __device__ __forceinline__ void plane_step(int mode, float str, float off, float delz,
                                            float& coord, float& mom, float gz, float igz,
-                                           bool poly) {
+                                           int poly) {
   if (mode == 0) {
     coord += mom * delz * igz;
     return;
@@ -136,16 +138,22 @@ __device__ __forceinline__ void plane_step(int mode, float str, float off, float
   if (poly) {
     const float k = str * igz;
     const float w = k * delz * delz;
-    float c = 1.0f / 40320.0f;  // cos(sqrt(w)) = 1 - w/2 + w^2/24 - ...
-    c = fmaf(c, w, -1.0f / 720.0f);
-    c = fmaf(c, w, 1.0f / 24.0f);
-    c = fmaf(c, w, -0.5f);
-    c = fmaf(c, w, 1.0f);
-    float sn = 1.0f / 362880.0f;  // sin(sqrt(w))/sqrt(w) = 1 - w/6 + w^2/120 - ...
-    sn = fmaf(sn, w, -1.0f / 5040.0f);
-    sn = fmaf(sn, w, 1.0f / 120.0f);
-    sn = fmaf(sn, w, -1.0f / 6.0f);
-    sn = fmaf(sn, w, 1.0f);
+    float c, sn;
+    if (poly == 2) {
+      c = fmaf(fmaf(w, 1.0f / 24.0f, -0.5f), w, 1.0f);  // cos(sqrt(w)) = 1 - w/2 + w^2/24
+      sn = fmaf(fmaf(w, 1.0f / 120.0f, -1.0f / 6.0f), w, 1.0f);  // 1 - w/6 + w^2/120
+    } else {
+      c = 1.0f / 40320.0f;  // cos(sqrt(w)) = 1 - w/2 + w^2/24 - ...
+      c = fmaf(c, w, -1.0f / 720.0f);
+      c = fmaf(c, w, 1.0f / 24.0f);
+      c = fmaf(c, w, -0.5f);
+      c = fmaf(c, w, 1.0f);
+      sn = 1.0f / 362880.0f;  // sin(sqrt(w))/sqrt(w) = 1 - w/6 + w^2/120 - ...
+      sn = fmaf(sn, w, -1.0f / 5040.0f);
+      sn = fmaf(sn, w, 1.0f / 120.0f);
+      sn = fmaf(sn, w, -1.0f / 6.0f);
+      sn = fmaf(sn, w, 1.0f);
+    }
     const float a2 = delz * sn;  // sin(omega)/foc
     const float xt = coord - off;
     coord = fmaf(c, xt, a2 * mom * igz) + off;
@@ -187,13 +195,14 @@ __device__ __forceinline__ double sconst(double c) {
   return r;
 }
 
-// FP64 with `poly`: the power series of the MIXED overload below taken to
-// w^8 (for |w| <= 1/2 the remainders are below 1e-18, under FP64
-// rounding), with one reciprocal of gammaz for both planes; otherwise the
-// reference's trigonometric form.
+// FP64 with `poly` > 0: the power series of the MIXED overload below, to
+// w^8 for |w| <= 1/2 (poly 1; remainder w^9/18! < 4e-19) or to w^4 for
+// |w| <= 0.011 (poly 2; w^5/10! < 5e-17), either under FP64 rounding, with
+// one reciprocal of gammaz for both planes; otherwise the reference's
+// trigonometric form.
 __device__ __forceinline__ void plane_step(int mode, double str, double off, double delz,
                                            double& coord, double& mom, double gz, double igz,
-                                           bool poly) {
+                                           int poly) {
   if (mode == 0) {
     coord += mom * delz * igz;
     return;
@@ -204,24 +213,36 @@ __device__ __forceinline__ void plane_step(int mode, double str, double off, dou
   }
   const double k = str * igz;
   const double w = k * delz * delz;
-  double c = sconst(1.0 / 20922789888000.0);  // cos(sqrt(w)): (-w)^n / (2n)!, n <= 8
-  c = fma(c, w, sconst(-1.0 / 87178291200.0));
-  c = fma(c, w, sconst(1.0 / 479001600.0));
-  c = fma(c, w, sconst(-1.0 / 3628800.0));
-  c = fma(c, w, sconst(1.0 / 40320.0));
-  c = fma(c, w, sconst(-1.0 / 720.0));
-  c = fma(c, w, sconst(1.0 / 24.0));
-  c = fma(c, w, -0.5);
-  c = fma(c, w, 1.0);
-  double sn = sconst(1.0 / 355687428096000.0);  // sin(sqrt(w))/sqrt(w): (-w)^n / (2n+1)!
-  sn = fma(sn, w, sconst(-1.0 / 1307674368000.0));
-  sn = fma(sn, w, sconst(1.0 / 6227020800.0));
-  sn = fma(sn, w, sconst(-1.0 / 39916800.0));
-  sn = fma(sn, w, sconst(1.0 / 362880.0));
-  sn = fma(sn, w, sconst(-1.0 / 5040.0));
-  sn = fma(sn, w, sconst(1.0 / 120.0));
-  sn = fma(sn, w, sconst(-1.0 / 6.0));
-  sn = fma(sn, w, 1.0);
+  double c, sn;
+  if (poly == 2) {
+    c = fma(sconst(1.0 / 40320.0), w, sconst(-1.0 / 720.0));  // (-w)^n / (2n)!, n <= 4
+    c = fma(c, w, sconst(1.0 / 24.0));
+    c = fma(c, w, -0.5);
+    c = fma(c, w, 1.0);
+    sn = fma(sconst(1.0 / 362880.0), w, sconst(-1.0 / 5040.0));  // (-w)^n / (2n+1)!
+    sn = fma(sn, w, sconst(1.0 / 120.0));
+    sn = fma(sn, w, sconst(-1.0 / 6.0));
+    sn = fma(sn, w, 1.0);
+  } else {
+    c = sconst(1.0 / 20922789888000.0);  // cos(sqrt(w)): (-w)^n / (2n)!, n <= 8
+    c = fma(c, w, sconst(-1.0 / 87178291200.0));
+    c = fma(c, w, sconst(1.0 / 479001600.0));
+    c = fma(c, w, sconst(-1.0 / 3628800.0));
+    c = fma(c, w, sconst(1.0 / 40320.0));
+    c = fma(c, w, sconst(-1.0 / 720.0));
+    c = fma(c, w, sconst(1.0 / 24.0));
+    c = fma(c, w, -0.5);
+    c = fma(c, w, 1.0);
+    sn = sconst(1.0 / 355687428096000.0);  // sin(sqrt(w))/sqrt(w): (-w)^n / (2n+1)!
+    sn = fma(sn, w, sconst(-1.0 / 1307674368000.0));
+    sn = fma(sn, w, sconst(1.0 / 6227020800.0));
+    sn = fma(sn, w, sconst(-1.0 / 39916800.0));
+    sn = fma(sn, w, sconst(1.0 / 362880.0));
+    sn = fma(sn, w, sconst(-1.0 / 5040.0));
+    sn = fma(sn, w, sconst(1.0 / 120.0));
+    sn = fma(sn, w, sconst(-1.0 / 6.0));
+    sn = fma(sn, w, 1.0);
+  }
   const double a2 = delz * sn;  // sin(omega)/foc
   const double xt = coord - off;
   coord = fma(c, xt, a2 * mom * igz) + off;
@@ -309,16 +330,14 @@ __device__ __forceinline__ void transverse(const TransParams& t, double g0, P g,
     const double gzz = gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py;
     const double igz = rsq_nr(gzz);  // 1/gammaz
     const double gz = gzz * igz;
-    plane_step(t.xmode, t.xstr, t.xoff, t.delz, x, px, gz, igz, t.xpoly != 0);
-    plane_step(t.ymode, t.ystr, t.yoff, t.delz, y, py, gz, igz, t.ypoly != 0);
+    plane_step(t.xmode, t.xstr, t.xoff, t.delz, x, px, gz, igz, t.xpoly);
+    plane_step(t.ymode, t.ystr, t.yoff, t.delz, y, py, gz, igz, t.ypoly);
   } else {
     const float gzz = gz2(g, g0, t.g2c, t.aw2, true) - px * px - py * py;
     const float igz = __builtin_amdgcn_rsqf(gzz);  // 1/gammaz
     const float gz = gzz * igz;
-    plane_step(t.xmode, float(t.xstr), float(t.xoff), float(t.delz), x, px, gz, igz,
-               t.xpoly != 0);
-    plane_step(t.ymode, float(t.ystr), float(t.yoff), float(t.delz), y, py, gz, igz,
-               t.ypoly != 0);
+    plane_step(t.xmode, float(t.xstr), float(t.xoff), float(t.delz), x, px, gz, igz, t.xpoly);
+    plane_step(t.ymode, float(t.ystr), float(t.yoff), float(t.delz), y, py, gz, igz, t.ypoly);
   }
 }
 
@@ -683,12 +702,14 @@ __device__ __forceinline__ double wave_max(double v) {
 // mode and, in MIXED mode, FP32 over the thread's own particles (a slice's
 // particles / kThreads of them); every reduction across threads, waves and
 // slices is FP64.
-// quad_perm DPP: lane i takes lane i ^ 1 / i ^ 2 of its quad
+// quad_perm DPP: lane i takes lane i ^ 1 / i ^ 2 of its quad.  Callers run
+// them with whole quads active; bound_ctrl (zero from an inactive source)
+// leaves the old value unused, so the compiler needs no copy of v for it.
 __device__ __forceinline__ int quad_xor1(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int quad_xor2(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float quad_xor1(float v) {
   return __builtin_bit_cast(float, quad_xor1(__builtin_bit_cast(int, v)));
@@ -878,7 +899,7 @@ __device__ __forceinline__ void process(bool diag_on, const ParticleParams& prm,
       bool done = false;
       if (__builtin_amdgcn_read_exec() == ~0ull) {  // whole wave here: quads are complete
         const int key = fast ? lx0 + ly0 * ww : -1 - static_cast<int>(threadIdx.x & 63);
-        int same = (key == quad_xor1(key) && key == quad_xor2(key)) ? 1 : 0;
+        int same = (key == quad_xor1(key) ? 1 : 0) & (key == quad_xor2(key) ? 1 : 0);
         same &= quad_xor1(same) & quad_xor2(same);
         if (same) {
           P v[8];

@@ -56,10 +56,17 @@ def noisy_field(c, seed=7, scale=1.0):
     return re, im
 
 
+# K dz^2 of the half step, bounded for gamma >= gamma0/2: 5e-4 at qf = 1.5
+# (the short series of the plane maps), 0.034 / 0.10 at qf = 100 / -300 (the
+# long series), 1.7 at qf = 5000 (the trigonometric form)
 TRANSVERSE_STEPS = {
     "undulator": dict(),
     "quad_focus": dict(qf=1.5),
     "quad_defocus": dict(qf=-1.5),
+    "quad_medium": dict(qf=100.0),
+    "quad_medium_defocus": dict(qf=-300.0),
+    "quad_strong": dict(qf=5000.0),
+    "quad_strong_defocus": dict(qf=-5000.0),
     "drift": dict(aw=0.0, in_und=False),
     "offsets": dict(qf=1.5, dqx=1e-5, dqy=-2e-5, ax=3e-6, ay=-4e-6),
     "corrector": dict(cx=1e-6, cy=-2e-6),
@@ -82,6 +89,25 @@ def test_track_transverse_fp64(gpu_ctx, name, last):
     for a, r in ((x, o.x), (y, o.y), (px, o.px), (py, o.py)):
         assert relerr(a, r) < 1e-13
     assert np.array_equal(g, o.gamma) and np.array_equal(t, o.theta)
+
+
+@pytest.mark.parametrize("name", ["undulator", "quad_focus", "quad_medium_defocus",
+                                  "quad_strong", "chicane"])
+def test_track_transverse_mixed(gpu_ctx, name):
+    """MIXED transverse maps (FP32 coordinates; short and long series, the
+    trigonometric form, the chicane) against the FP64 oracle: normwise 1e-6,
+    a few FP32 roundings of each coordinate."""
+    print(f"case track_transverse_mixed {name}", flush=True)
+    c = cases.make_case(nslice=4, npart=512)
+    st = quad_step(**TRANSVERSE_STEPS[name])
+    b = gpu_beam(gpu_ctx, c, api.MIXED)
+    b.track_transverse(st, st.delz * 0.5, False)
+    x, y, px, py, g, t = b.download()
+    o = c.beam.copy()
+    _oracle.lib().orc_track_transverse(o.x.size, dp(o.x), dp(o.y), dp(o.px), dp(o.py),
+                                       dp(o.gamma), st, GAMMA0, st.delz * 0.5, 0)
+    for a, r in ((x, o.x), (y, o.y), (px, o.px), (py, o.py)):
+        assert relerr(a, r) < 1e-6
 
 
 def test_r56_fp64(gpu_ctx):
