@@ -421,29 +421,47 @@ __device__ __forceinline__ void gather(const GatherField& f, int slice, P x, P y
 // full sincos per step and field serves all of them.  Taylor series of sin
 // (to d^13) and cos (to d^14) in Horner form; for |d| <= 1/2 the remainders
 // are below 3e-17, i.e. the result is the direct sincos(a0 + d) to a few
-// ulp.  Larger offsets take the full routine.
+// ulp.  When every lane of the wave has |d| <= 1/16 (a phase advance of a
+// step well inside the FEL bandwidth) the series stop at d^9 and d^8: the
+// first omitted terms, d^11/11! < 2e-21 and d^10/10! < 3e-19, are as far
+// below FP64 rounding.  Larger offsets take the full routine.
+constexpr bool kShortNear = true;
 __device__ __forceinline__ void sincos_near(double d, double a, double s0, double c0, double& s,
                                             double& c) {
-  if (fabs(d) > 0.5) {
-    sincos(a, &s, &c);
-    return;
-  }
   const double d2 = d * d;
-  double ps = sconst(-1.0 / 6227020800.0);  // -1/13!
-  ps = fma(ps, d2, sconst(1.0 / 39916800.0));
-  ps = fma(ps, d2, sconst(-1.0 / 362880.0));
-  ps = fma(ps, d2, sconst(1.0 / 5040.0));
-  ps = fma(ps, d2, sconst(-1.0 / 120.0));
-  ps = fma(ps, d2, sconst(1.0 / 6.0));
-  const double sd = fma(-ps * d2, d, d);  // d - d^3/3! + ...
-  double pc = sconst(1.0 / 87178291200.0);  // 1/14!
-  pc = fma(pc, d2, sconst(-1.0 / 479001600.0));
-  pc = fma(pc, d2, sconst(1.0 / 3628800.0));
-  pc = fma(pc, d2, sconst(-1.0 / 40320.0));
-  pc = fma(pc, d2, sconst(1.0 / 720.0));
-  pc = fma(pc, d2, sconst(-1.0 / 24.0));
-  pc = fma(pc, d2, 0.5);
-  const double cd = fma(-pc, d2, 1.0);  // 1 - d^2/2! + ...
+  double sd, cd;
+  if (kShortNear && __builtin_amdgcn_ballot_w64(!(fabs(d) <= 0.0625)) == 0) {
+    double ps = sconst(1.0 / 362880.0);  // 1/9!
+    ps = fma(ps, d2, sconst(-1.0 / 5040.0));
+    ps = fma(ps, d2, sconst(1.0 / 120.0));
+    ps = fma(ps, d2, sconst(-1.0 / 6.0));
+    sd = fma(ps * d2, d, d);  // d - d^3/3! + ...
+    double pc = sconst(1.0 / 40320.0);  // 1/8!
+    pc = fma(pc, d2, sconst(-1.0 / 720.0));
+    pc = fma(pc, d2, sconst(1.0 / 24.0));
+    pc = fma(pc, d2, -0.5);
+    cd = fma(pc, d2, 1.0);  // 1 - d^2/2! + ...
+  } else {
+    if (fabs(d) > 0.5) {
+      sincos(a, &s, &c);
+      return;
+    }
+    double ps = sconst(-1.0 / 6227020800.0);  // -1/13!
+    ps = fma(ps, d2, sconst(1.0 / 39916800.0));
+    ps = fma(ps, d2, sconst(-1.0 / 362880.0));
+    ps = fma(ps, d2, sconst(1.0 / 5040.0));
+    ps = fma(ps, d2, sconst(-1.0 / 120.0));
+    ps = fma(ps, d2, sconst(1.0 / 6.0));
+    sd = fma(-ps * d2, d, d);  // d - d^3/3! + ...
+    double pc = sconst(1.0 / 87178291200.0);  // 1/14!
+    pc = fma(pc, d2, sconst(-1.0 / 479001600.0));
+    pc = fma(pc, d2, sconst(1.0 / 3628800.0));
+    pc = fma(pc, d2, sconst(-1.0 / 40320.0));
+    pc = fma(pc, d2, sconst(1.0 / 720.0));
+    pc = fma(pc, d2, sconst(-1.0 / 24.0));
+    pc = fma(pc, d2, 0.5);
+    cd = fma(-pc, d2, 1.0);  // 1 - d^2/2! + ...
+  }
   s = fma(s0, cd, c0 * sd);
   c = fma(c0, cd, -s0 * sd);
 }
@@ -453,7 +471,7 @@ __device__ __forceinline__ void sincos_near(double d, double a, double s0, doubl
 // factors exp(-i h theta) come from the step's initial ones (bs, bc at
 // theta0 = tthet - dth) through sincos_near, and 1/beta_par through
 // inv_btpar, rounded as the reference rounds it.
-template <bool ACC, int NF>
+template <bool ACC, int NF, bool FIRST = false>
 __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, double tthet,
                                       double dth, const double* bs, const double* bc, double btpar,
                                       double ez, const double* rr, const double* ri,
@@ -466,7 +484,12 @@ __device__ __forceinline__ void ode64(const PushParams& p, int nf, double tgam, 
   for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
     if (NF == 0 && i >= nf) break;
     double s, c;
-    sincos_near(p.f[i].rharm * dth, p.f[i].rharm * tthet, bs[i], bc[i], s, c);
+    if (FIRST) {  // the first stage is at the initial phase itself
+      s = bs[i];
+      c = bc[i];
+    } else {
+      sincos_near(p.f[i].rharm * dth, p.f[i].rharm * tthet, bs[i], bc[i], s, c);
+    }
     if (i == 0) {
       cre = rr[i] * c + ri[i] * s;
       cim = ri[i] * c - rr[i] * s;
@@ -574,7 +597,7 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
     sincos(p.f[i].rharm * t0, &bs[i], &bc[i]);
   }
   double k2gg, k2pp, k3gg, k3pp;
-  ode64<false, NF>(p, nf, g, t, 0.0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
+  ode64<false, NF, true>(p, nf, g, t, 0.0, bs, bc, btpar, ez, rr, ri, k2gg, k2pp);
   double stpz = 0.5 * p.delz;
   g += stpz * k2gg;
   t += stpz * k2pp;
