@@ -729,6 +729,7 @@ g4h_status g4h_field_init_solver(g4h_field* f, g4h_solver_kind kind, int filter,
   f->filter = kind == G4H_SOLVER_FFT && filter && !(sig <= 0 || xc <= 0 || yc <= 0);
   G4_HIP(hipStreamSynchronize(f->ctx->stream));  // tables may still be read
   f->fft_tw.release();  // tables are rebuilt on the next init
+  f->fft_tw_conv.release();
   f->fft_sigmoid.release();
   f->delz_tables.clear();
   ++f->generation;  // device tables of the solver are gone

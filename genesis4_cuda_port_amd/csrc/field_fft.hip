@@ -618,7 +618,213 @@ __global__ void gather_source_kernel(const typename C2<P>::T* src, const int4* r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Other sizes N with the propagator: convolution through power-of-two
+// transforms.  One line's step IDFT_N(P . DFT_N(u)) is the circular
+// convolution of u with h = IDFT_N(P); with g_j = h_((j - N + 1) mod N) for
+// j <= 2N - 2 it is the linear convolution (u * g) at N - 1 ... 2N - 2, and a
+// circular convolution of M >= 2N - 1 points gives those entries without
+// wrap-around.  So each line is padded with zeros to M points, transformed
+// forward, multiplied by H = DFT_M(g) / M (the host's table, in pi order) and
+// transformed back, and points N - 1 ... 2N - 2 are its result: two M-point
+// register transforms instead of two N-point direct DFTs (O(M log M) instead
+// of O(N^2) per line).
+// ---------------------------------------------------------------------------
+
+// Row pass: rows of N points, each held on the lanes of a line slot as in
+// fft_rows_reg_kernel (without its prefetch: the padded lines already take
+// twice the registers).
+template <class P, int M>
+__global__ __launch_bounds__((RegCfg<P, M>::kThreads)) void fft_rows_conv_kernel(
+    typename C2<P>::T* u, int64_t nrows, int N, const double2* tw_d, const double2* mult_d) {
+  using T2 = typename C2<P>::T;
+  using S = fftr::Shape<M>;
+  using C = RegCfg<P, M>;
+  constexpr int R = S::R, L = S::L;
+  __shared__ T2 tw[M];
+  __shared__ T2 mu[M];
+  __shared__ T2 buf[C::kWaves * S::LPW * S::BUF];
+  load_table(tw, tw_d, M);
+  load_table(mu, mult_d, M);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L, t = lane % L;
+  T2* lbuf = buf + (wave * S::LPW + g) * S::BUF;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * C::kWaves;
+  for (int64_t grp = blockIdx.x * static_cast<int64_t>(C::kWaves) + wave; grp * S::LPW < nrows;
+       grp += nwaves) {
+    const int64_t line = grp * S::LPW + g;
+    const bool valid = line < nrows;
+    T2* row = u + line * N;
+    T2 v[R];
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      const int j = t + L * r;
+      v[r].x = 0;
+      v[r].y = 0;
+      if (valid && j < N) v[r] = ld_stream(&row[j]);
+    });
+    fftr::forward<T2, M>(v, t, tw, lbuf);
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int q = decltype(ic)::value;
+      v[q] = fftr::cmul(v[q], mu[q * L + t]);
+    });
+    fftr::inverse<T2, M>(v, t, tw, lbuf);
+    if (valid) {
+      fftr::static_for<0, R>([&](auto ic) {
+        constexpr int r = decltype(ic)::value;
+        const int j = t + L * r - (N - 1);
+        if (j >= 0 && j < N) st_stream(&row[j], v[r]);
+      });
+    }
+  }
+}
+
+// Column pass: a workgroup owns TW adjacent columns of one ring plane (the
+// last tile clipped at N), staged through LDS as fft_cols_reg_kernel stages
+// its tile (row segments per instruction, swizzled columns), transformed as
+// above, and written back with 2S added inside the slice's source rectangle
+// (or the filtered source of the work plane) and the field sums of the tile.
+template <class P, int M>
+struct ConvCfg {
+  using C = RegCfg<P, M>;
+  static constexpr int TW = C::kTile;  // power of two
+  static constexpr int kBuf = C::kWaves * fftr::Shape<M>::LPW * fftr::Shape<M>::BUF;
+  static constexpr int kColBuf = M * TW > kBuf ? M * TW : kBuf;  // covers N * TW (N < M)
+};
+
+template <class P, int M>
+__global__ __launch_bounds__((RegCfg<P, M>::kThreads)) void fft_cols_conv_kernel(
+    typename C2<P>::T* u, int nslice, int N, const double2* tw_d, const double2* mult_d,
+    const typename C2<P>::T* src, const int4* rect, int first, int add_src, const double2* work,
+    double* partials) {
+  using T2 = typename C2<P>::T;
+  using S = fftr::Shape<M>;
+  using C = RegCfg<P, M>;
+  using K = ConvCfg<P, M>;
+  constexpr int R = S::R, L = S::L, TW = K::TW;
+  constexpr int NT = C::kThreads;
+  __shared__ T2 tw[M];
+  __shared__ T2 mu[M];
+  __shared__ T2 buf[K::kColBuf];
+  __shared__ double red[C::kWaves][8];
+  load_table(tw, tw_d, M);
+  load_table(mu, mult_d, M);
+  const int ntile = (N + TW - 1) / TW;
+  const int p = blockIdx.x / ntile;
+  const int tile = blockIdx.x - p * ntile;
+  const int c0 = tile * TW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L, t = lane % L;
+  const int cslot = wave * S::LPW + g;  // line slot; slots >= TW transform zeros
+  T2* plane = u + static_cast<int64_t>(p) * N * N;
+  auto swz = [](int row, int col) { return row * TW + (col ^ (row & (TW - 1))); };
+  const int sc_col = threadIdx.x % TW;
+  const int col = c0 + sc_col;
+  const bool colok = col < N;
+#pragma unroll 4
+  for (int row = threadIdx.x / TW; row < N; row += NT / TW) {
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+    if (colok) z = ld_stream(&plane[static_cast<int64_t>(row) * N + col]);
+    buf[swz(row, sc_col)] = z;
+  }
+  __syncthreads();
+  T2 v[R];
+  fftr::static_for<0, R>([&](auto ic) {
+    constexpr int r = decltype(ic)::value;
+    const int j = t + L * r;
+    v[r].x = 0;
+    v[r].y = 0;
+    if (cslot < TW && j < N) v[r] = buf[swz(j, cslot)];
+  });
+  __syncthreads();
+  T2* lbuf = buf + cslot * S::BUF;
+  fftr::forward<T2, M>(v, t, tw, lbuf);
+  fftr::static_for<0, R>([&](auto ic) {
+    constexpr int q = decltype(ic)::value;
+    v[q] = fftr::cmul(v[q], mu[q * L + t]);
+  });
+  fftr::inverse<T2, M>(v, t, tw, lbuf);
+  __syncthreads();
+  fftr::static_for<0, R>([&](auto ic) {
+    constexpr int r = decltype(ic)::value;
+    const int j = t + L * r - (N - 1);
+    if (cslot < TW && j >= 0 && j < N) buf[swz(j, cslot)] = v[r];
+  });
+  __syncthreads();
+
+  const int s = ((p - first) % nslice + nslice) % nslice;  // beam slice of this ring plane
+  int4 rc = make_int4(0, 0, 0, 0);
+  if (add_src == 1) rc = rect[s];
+  const T2* sp = (add_src == 1) ? src + static_cast<int64_t>(s) * N * N : nullptr;
+  const double2* wp = (add_src == 2) ? work + static_cast<int64_t>(p) * N * N : nullptr;
+  const bool scol = sp && col >= rc.x && col < rc.z;
+  const double shift = -0.5 * static_cast<double>(N - 1);
+  const int64_t icenter = (static_cast<int64_t>(N) * N - 1) / 2;
+  const double dx = col + shift;
+  double sw = 0.0, acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (colok) {
+    for (int row = threadIdx.x / TW; row < N; row += NT / TW) {
+      const int64_t gi = static_cast<int64_t>(row) * N + col;
+      T2 o = buf[swz(row, sc_col)];
+      if (scol && row >= rc.y && row < rc.w) {
+        const T2 sv = ld_stream(&sp[gi]);
+        o.x += P(2) * sv.x;
+        o.y += P(2) * sv.y;
+      } else if (wp) {
+        const double2 wv = ld_stream(&wp[gi]);
+        o.x += static_cast<P>(wv.x);
+        o.y += static_cast<P>(wv.y);
+      }
+      st_stream(&plane[gi], o);
+      if (partials) {
+        const double re = o.x, im = o.y;
+        const double wei = re * re + im * im;
+        const double dy = row + shift;
+        sw += wei;
+        acc[3] += dy * wei;
+        acc[4] += dy * dy * wei;
+        acc[5] += re;
+        acc[6] += im;
+        if (gi == icenter) {
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
+          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+        }
+      }
+    }
+  }
+  if (partials) {
+    acc[0] = sw;
+    acc[1] = dx * sw;
+    acc[2] = dx * dx * sw;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      double a = acc[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      if (lane == 0) red[wave][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double a = 0.0;
+      for (int w = 0; w < C::kWaves; ++w) a += red[w][threadIdx.x];
+      partials[(static_cast<int64_t>(p) * ntile + tile) * 8 + threadIdx.x] = a;
+    }
+  }
+}
+
 bool is_pow2(int g) { return g >= 2 && (g & (g - 1)) == 0; }
+
+// Points M of the power-of-two transforms that step a line of G points by
+// convolution (sizes that are not powers of two, up to 512), else 0.
+int conv_points(int G) {
+  if (G < 3 || G > 512 || is_pow2(G)) return 0;
+  int m = 2;
+  while (m < 2 * G - 1) m *= 2;
+  return m;
+}
 
 // Launch shape of the DFT kernels (sizes that are not powers of two).
 struct DftCfg {
@@ -663,6 +869,31 @@ void cols_reg(Field* f, typename C2<P>::T* data, int mode, double scale, int add
                      f->src_rect.get(), f->first, add_src, f->fft_work.get(), partials);
 }
 
+template <class P, int M>
+void rows_conv(Field* f, typename C2<P>::T* data) {
+  using C = RegCfg<P, M>;
+  constexpr int LPW = fftr::Shape<M>::LPW;
+  const int N = f->ngrid;
+  const int64_t nrows = static_cast<int64_t>(f->nslice) * N;
+  const int64_t ngroups = (nrows + LPW - 1) / LPW;
+  const int64_t nblk = std::min<int64_t>((ngroups + C::kWaves - 1) / C::kWaves, 8192);
+  hipLaunchKernelGGL((fft_rows_conv_kernel<P, M>), dim3(static_cast<unsigned>(nblk)),
+                     dim3(C::kThreads), 0, f->ctx->stream, data, nrows, N, f->fft_tw_conv.get(),
+                     f->fft_prop);
+}
+
+template <class P, int M>
+void cols_conv(Field* f, typename C2<P>::T* data, int add_src, double* partials) {
+  using T2 = typename C2<P>::T;
+  using C = RegCfg<P, M>;
+  const int N = f->ngrid;
+  const int ntile = (N + ConvCfg<P, M>::TW - 1) / ConvCfg<P, M>::TW;
+  hipLaunchKernelGGL((fft_cols_conv_kernel<P, M>), dim3(f->nslice * ntile), dim3(C::kThreads), 0,
+                     f->ctx->stream, data, f->nslice, N, f->fft_tw_conv.get(), f->fft_prop,
+                     static_cast<const T2*>(f->src()), f->src_rect.get(), f->first, add_src,
+                     f->fft_work.get(), partials);
+}
+
 // Calls fn with the compile-time size G (powers of two 2..1024).
 template <class Fn>
 void with_pow2(int G, Fn&& fn) {
@@ -683,6 +914,14 @@ void with_pow2(int G, Fn&& fn) {
 
 // Columns per workgroup tile of the column pass.
 int col_tile(int G, size_t esz) {
+  if (const int M = conv_points(G)) {
+    int tc = 0;
+    with_pow2(M, [&](auto m) {
+      constexpr int mv = decltype(m)::value;
+      tc = esz == sizeof(double2) ? ConvCfg<double, mv>::TW : ConvCfg<float, mv>::TW;
+    });
+    return tc;
+  }
   if (!is_pow2(G)) return dft_cfg(G, esz).tc;
   int tc = 0;
   with_pow2(G, [&](auto g) {
@@ -699,6 +938,10 @@ void run_rows(Field* f, typename C2<P>::T* data, int mode, double scale) {
   f->ctx->launch(G4H_K_FFT_ROWS, [&] {
     if (is_pow2(G)) {
       with_pow2(G, [&](auto g) { rows_reg<P, decltype(g)::value>(f, data, mode, scale); });
+      return;
+    }
+    if (mode == ROW_PROP && conv_points(G)) {  // the table holds the normalisation
+      with_pow2(conv_points(G), [&](auto m) { rows_conv<P, decltype(m)::value>(f, data); });
       return;
     }
     const DftCfg c = dft_cfg(G, sizeof(T2));
@@ -720,6 +963,12 @@ void run_cols(Field* f, typename C2<P>::T* data, int mode, double scale, int add
     if (is_pow2(G)) {
       with_pow2(G, [&](auto g) {
         cols_reg<P, decltype(g)::value>(f, data, mode, scale, add_src, partials);
+      });
+      return;
+    }
+    if (mode == COL_PROP && conv_points(G)) {
+      with_pow2(conv_points(G), [&](auto m) {
+        cols_conv<P, decltype(m)::value>(f, data, add_src, partials);
       });
       return;
     }
@@ -767,6 +1016,51 @@ void fft_step(Field* f, bool source_on, bool stats) {
   if (stats) field_reduce_partials(f, partials, ntile);
 }
 
+// The M-point convolution table of a line step with factors P (natural
+// frequency order, N points): h = IDFT_N(P), g_j = h_((j - N + 1) mod N) for
+// j <= 2N - 2 (zero beyond), H = DFT_M(g) / M, stored in the pi order of the
+// M-point register transforms.  Sums in long double over exact twiddle
+// indices, so the table is correctly rounded to a few ulp.
+std::vector<double2> conv_spectrum(const std::vector<double2>& P, int M) {
+  using LD = long double;
+  const int N = static_cast<int>(P.size());
+  const LD pi2 = 8 * std::atan(static_cast<LD>(1));
+  std::vector<LD> cn(N), sn(N), cm(M), sm(M);
+  for (int k = 0; k < N; ++k) {
+    cn[k] = std::cos(pi2 * k / N);
+    sn[k] = std::sin(pi2 * k / N);
+  }
+  for (int k = 0; k < M; ++k) {
+    cm[k] = std::cos(pi2 * k / M);
+    sm[k] = std::sin(pi2 * k / M);
+  }
+  std::vector<LD> hr(N), hi(N);
+  for (int n = 0; n < N; ++n) {  // h_n = (1/N) sum_m P_m e^{+2 pi i m n / N}
+    LD ar = 0, ai = 0;
+    for (int m = 0; m < N; ++m) {
+      const int q = static_cast<int>((static_cast<int64_t>(m) * n) % N);
+      ar += P[m].x * cn[q] - P[m].y * sn[q];
+      ai += P[m].x * sn[q] + P[m].y * cn[q];
+    }
+    hr[n] = ar / N;
+    hi[n] = ai / N;
+  }
+  std::vector<double2> out(M);
+  for (int k = 0; k < M; ++k) {  // H_k = (1/M) sum_j g_j e^{-2 pi i j k / M}
+    LD ar = 0, ai = 0;
+    for (int j = 0; j <= 2 * N - 2; ++j) {
+      const int n = (j - N + 1 + N) % N;
+      const int q = static_cast<int>((static_cast<int64_t>(j) * k) % M);
+      ar += hr[n] * cm[q] + hi[n] * sm[q];
+      ai += hi[n] * cm[q] - hr[n] * sm[q];
+    }
+    out[k] = make_double2(static_cast<double>(ar / M), static_cast<double>(ai / M));
+  }
+  std::vector<double2> perm(M);
+  for (int e = 0; e < M; ++e) perm[e] = out[fftr::spectrum_index(e, M)];
+  return perm;
+}
+
 }  // namespace
 
 void fft_prepare(Field* f, double delz) {
@@ -787,6 +1081,19 @@ void fft_prepare(Field* f, double delz) {
     }
     f->fft_tw.alloc(f->ctx, G);
     G4_HIP(hipMemcpy(f->fft_tw.get(), tw.data(), sizeof(double2) * G, hipMemcpyHostToDevice));
+    if (const int M = conv_points(G)) {
+      // the M-point convolution transforms' four-step twiddles
+      const int L = fftr::lanes_per_line(M);
+      std::vector<double2> twm(M);
+      for (int k = 0; k < M; ++k) {
+        const int64_t e = static_cast<int64_t>(k % L) * (k / L);
+        const double a = -pi2 * static_cast<double>(e % M) / static_cast<double>(M);
+        twm[k] = make_double2(std::cos(a), std::sin(a));
+      }
+      f->fft_tw_conv.alloc(f->ctx, M);
+      G4_HIP(hipMemcpy(f->fft_tw_conv.get(), twm.data(), sizeof(double2) * M,
+                       hipMemcpyHostToDevice));
+    }
     f->fft_delz = 0.0;
     f->fft_prop = nullptr;
     if (f->filter) {
@@ -829,8 +1136,9 @@ void fft_prepare(Field* f, double delz) {
         const double ph = -dx * dx * dk * dk / 2. / f->xks * delz;
         t.host[e] = make_double2(std::cos(ph), std::sin(ph));
       }
-      t.dev.alloc(f->ctx, G);
-      G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * G,
+      if (const int M = conv_points(G)) t.host = conv_spectrum(t.host, M);
+      t.dev.alloc(f->ctx, t.host.size());
+      G4_HIP(hipMemcpyAsync(t.dev.get(), t.host.data(), sizeof(double2) * t.host.size(),
                             hipMemcpyHostToDevice, f->ctx->stream));
       it = f->delz_tables.emplace(delz, std::move(t)).first;
     }

@@ -321,7 +321,7 @@ struct Field {
   // lattice: built once, uploaded asynchronously (the host copy lives with
   // the entry), selected per step without a host synchronisation.
   struct DelzTable {
-    std::vector<double2> host;  // [3G] ADI c, cbet, cwet, or [G] FFT factor
+    std::vector<double2> host;  // [3G] ADI c, cbet, cwet, or [G] FFT factor ([M] convolution)
     DevBuf<double2> dev;
   };
   std::map<double, DelzTable> delz_tables;
@@ -351,7 +351,9 @@ struct Field {
   // FFT
   double fft_delz = 0.0;
   DevBuf<double2> fft_tw;            // four-step twiddles / DFT twiddles, [G]
-  const double2* fft_prop = nullptr;  // current separable propagator factor [G]
+  DevBuf<double2> fft_tw_conv;       // four-step twiddles of the M-point convolution transforms
+  const double2* fft_prop = nullptr;  // current separable propagator factor [G], or its
+                                      // convolution spectrum [M] (conv_points)
   DevBuf<double> fft_sigmoid;  // [G*G] (filter only)
   DevBuf<double2> fft_work;    // filtered-source work planes (filter only)
   // diagnostics

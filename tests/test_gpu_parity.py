@@ -248,19 +248,23 @@ def test_field_track_fp64(gpu_ctx, kind, ngrid):
 
 
 @pytest.mark.parametrize("prec,ngrid", [(api.FP64, g) for g in (4, 8, 16, 64, 128, 256, 512, 1024,
-                                                                   48, 96)]
-                         + [(api.MIXED, g) for g in (16, 64, 256, 1024)])
+                                                                   3, 5, 15, 33, 48, 96, 151, 255,
+                                                                   500, 600)]
+                         + [(api.MIXED, g) for g in (16, 64, 256, 1024, 33, 151, 255)])
 def test_fft_sizes(gpu_ctx, prec, ngrid):
     """Spectral step at every transform shape: one or several lines per wave
-    (G < 64), one to sixteen points per lane (G >= 64), and the direct DFT
-    of sizes that are not powers of two."""
+    (G < 64), one to sixteen points per lane (G >= 64); sizes that are not
+    powers of two up to 512 through the convolution by M-point transforms
+    (M = 8 ... 1024), larger ones (600) through the direct DFT."""
     print(f"case fft_sizes prec={prec} ngrid={ngrid}", flush=True)
-    c = cases.make_case(nslice=3, npart=256, ngrid=ngrid, seed_power=1e6)
+    ns = 1 if ngrid == 600 else 3  # the oracle's direct DFT at 600 takes seconds per plane
+    first = 2 % ns
+    c = cases.make_case(nslice=ns, npart=256, ngrid=ngrid, seed_power=1e6)
     st = quad_step()
     re, im = noisy_field(c, scale=0.1)
-    f = gpu_field(gpu_ctx, c, prec, kind=api.FFT, re=re, im=im, first=2)
+    f = gpu_field(gpu_ctx, c, prec, kind=api.FFT, re=re, im=im, first=first)
     b = gpu_beam(gpu_ctx, c, prec)
-    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, re, im, first=2)
+    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, re, im, first=first)
     o = c.beam
     tol = 1e-12 if prec == api.FP64 else 2e-5
     for it in range(2):
@@ -271,8 +275,8 @@ def test_fft_sizes(gpu_ctx, prec, ngrid):
                                   dp(o.theta), of.ptr, st, dp(o.current), st.delz, dp(sre),
                                   dp(sim))
         _oracle.lib().orc_fft_advance(of.ptr, dp(sre), dp(sim), st.delz, 0, 1.0, 1.0, 1.0)
-        gre, gim, first = f.download()
-        assert first == 2
+        gre, gim, ring = f.download()
+        assert ring == first
         e = relerr(gre + 1j * gim, of.re + 1j * of.im)
         print(f"  step {it}: rel err {e:.3e}", flush=True)
         assert e < tol
@@ -285,7 +289,7 @@ def test_fft_sizes(gpu_ctx, prec, ngrid):
     assert relerr(fraw, ofraw) < (1e-10 if prec == api.FP64 else 1e-4)
 
 
-@pytest.mark.parametrize("ngrid", [32, 128])
+@pytest.mark.parametrize("ngrid", [32, 128, 33])
 def test_fft_filter_fp64(gpu_ctx, ngrid):
     c = cases.make_case(nslice=3, npart=1024, ngrid=ngrid, seed_power=1e6)
     st = quad_step()
