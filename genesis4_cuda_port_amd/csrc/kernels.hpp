@@ -38,6 +38,23 @@ __device__ __forceinline__ void st_stream(double2* p, double2 v) {
   __builtin_nontemporal_store(t, reinterpret_cast<nt_d2*>(p));
 }
 
+// Loads through pointers that address global memory but that the compiler
+// cannot prove to (read from a parameter block in device memory): as flat
+// loads they would also count on the counter of LDS and scalar loads, so
+// every wait for a scalar load would wait for them too.
+typedef __attribute__((address_space(1))) const nt_f2* gl_f2;
+typedef __attribute__((address_space(1))) const nt_d2* gl_d2;
+typedef __attribute__((address_space(1))) const double* gl_d;
+__device__ __forceinline__ float2 ld_global(const float2* p) {
+  const nt_f2 v = *(gl_f2)(p);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_global(const double2* p) {
+  const nt_d2 v = *(gl_d2)(p);
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double ld_global(const double* p) { return *(gl_d)(p); }
+
 // ---- particle pass -------------------------------------------------------
 enum Stage : int {
   ST_T1 = 1,     // first transverse half step (with chicane)

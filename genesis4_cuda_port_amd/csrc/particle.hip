@@ -384,32 +384,49 @@ __device__ __forceinline__ bool cell_of(P x, P y, P gm, P dg, P idg, int ng, int
 }
 
 // ---- gather (BeamSolverCUDA.cpp:87-143) ---------------------------------
+// Split in two so that the caller can issue other loads between the cell
+// loads and their use: gather_load reads the four cells around the particle
+// (zeros off the grid), gather_sum weights them.
 template <class P>
-__device__ __forceinline__ void gather(const GatherField& f, int slice, P x, P y, P awloc, P& rr,
-                                       P& ri) {
+struct Gathered {
+  typename V2<P>::T a, b, c, d;
+  P wx, wy;
+};
+
+template <class P>
+__device__ __forceinline__ void gather_load(const GatherField& f, int slice, P x, P y,
+                                            Gathered<P>& q) {
   using T2 = typename V2<P>::T;
-  rr = P(0);
-  ri = P(0);
   const int ng = f.ngrid;
   int ix, iy;
-  P wx, wy;
-  if (!cell_of<P>(x, y, P(f.gridmax), P(f.dgrid), P(f.inv_dgrid), ng, ix, iy, wx, wy)) return;
+  q.a.x = q.a.y = q.b.x = q.b.y = q.c.x = q.c.y = q.d.x = q.d.y = P(0);
+  q.wx = q.wy = P(0);
+  if (!cell_of<P>(x, y, P(f.gridmax), P(f.dgrid), P(f.inv_dgrid), ng, ix, iy, q.wx, q.wy)) return;
   const int islice = (slice + f.first) % f.nslice;
   const T2* u = static_cast<const T2*>(f.data) + static_cast<int64_t>(islice) * ng * ng +
                 ix + static_cast<int64_t>(iy) * ng;
+  q.a = ld_global(&u[0]);
+  q.b = ld_global(&u[1]);
+  q.c = ld_global(&u[ng]);
+  q.d = ld_global(&u[ng + 1]);
+}
+
+template <class P>
+__device__ __forceinline__ void gather_sum(const GatherField& f, const Gathered<P>& q, P awloc,
+                                           P& rr, P& ri) {
+  const P wx = q.wx, wy = q.wy;
   const P w00 = wx * wy;
   const P w10 = (P(1) - wx) * wy;
   const P w01 = wx * (P(1) - wy);
   const P w11 = (P(1) - wx) * (P(1) - wy);
-  const T2 a = u[0], b = u[1], c = u[ng], d = u[ng + 1];
-  P cre = a.x * w00;
-  P cim = a.y * w00;
-  cre += b.x * w10;
-  cim += b.y * w10;
-  cre += c.x * w01;
-  cim += c.y * w01;
-  cre += d.x * w11;
-  cim += d.y * w11;
+  P cre = q.a.x * w00;
+  P cim = q.a.y * w00;
+  cre += q.b.x * w10;
+  cim += q.b.y * w10;
+  cre += q.c.x * w01;
+  cim += q.c.y * w01;
+  cre += q.d.x * w11;
+  cim += q.d.y * w11;
   const P scale = P(f.rtmp) * awloc;
   rr = scale * cre;
   ri = -scale * cim;
@@ -569,24 +586,43 @@ __device__ __forceinline__ void ode32(const PushParams& p, const MixedK& m, int 
 }
 
 // RK4 two-register scheme (BeamSolverCUDA.cpp:258-299), kept in order.
-template <int NF>
+template <int NF, class PF>
 __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t ip, double x,
                                        double y, double px, double py, double& gamma,
                                        double& theta, double& base_t, double& base_s,
-                                       double& base_c) {
+                                       double& base_c, bool on, PF&& pf) {
   const int nf = NF > 0 ? NF : p.nfield;
+  // The field cells first, then the caller's loads (the next particle): the
+  // memory counter retires in issue order, so waiting for the cells leaves
+  // the caller's loads in flight.  pf runs here whether or not the push is
+  // on (one call site: the loaded registers need no copies).
+  Gathered<double> q[NF > 0 ? NF : kMaxFields];
+  if (on) {
+#pragma unroll
+    for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+      if (NF == 0 && i >= nf) break;
+      gather_load<double>(p.f[i], slice, x, y, q[i]);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  pf();
+  __builtin_amdgcn_sched_barrier(0);
+  if (!on) return;
   double g = gamma;
   double t = theta + p.autophase;
   const double dx = x - p.ax, dy = y - p.ay;
   const double awloc = 1.0 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy;
   const double btpar = 1.0 + px * px + py * py + p.aw * p.aw * awloc * awloc;
   // load_longitudinal_ez (BeamSolverCUDA.cpp:183-191)
-  const double ez = p.particle_ez ? p.particle_ez[ip] : (p.slice_eloss ? p.slice_eloss[slice] : 0.0);
+  const double ez = p.particle_ez ? ld_global(&p.particle_ez[ip])
+                                  : (p.slice_eloss ? ld_global(&p.slice_eloss[slice]) : 0.0);
   double rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
+  {
 #pragma unroll
-  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
-    if (NF == 0 && i >= nf) break;
-    gather<double>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
+    for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+      if (NF == 0 && i >= nf) break;
+      gather_sum<double>(p.f[i], q[i], awloc, rr[i], ri[i]);
+    }
   }
   // phase factors at the step's initial phase, one full sincos per field
   const double t0 = t;
@@ -635,11 +671,24 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
 // reduced once per field; theta itself stays FP64 (theta0 + advance).
 // Outputs the reduced initial phase of the first field and the FP64 initial
 // phase for the sincos of the final phase.
-template <int NF>
+template <int NF, class PF>
 __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int slice, int64_t ip,
                                        float x, float y, float px, float py, float& dgamma,
-                                       double& theta, double& base_t, float& base_r) {
+                                       double& theta, double& base_t, float& base_r, bool on,
+                                       PF&& pf) {
   const int nf = NF > 0 ? NF : p.nfield;
+  Gathered<float> q[NF > 0 ? NF : kMaxFields];  // cells, then pf, as in push64
+  if (on) {
+#pragma unroll
+    for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+      if (NF == 0 && i >= nf) break;
+      gather_load<float>(p.f[i], slice, x, y, q[i]);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  pf();
+  __builtin_amdgcn_sched_barrier(0);
+  if (!on) return;
   float g = dgamma;
   const double t0 = theta + p.autophase;
   const float dx = x - float(p.ax), dy = y - float(p.ay);
@@ -648,13 +697,15 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   const float awloc = 1.0f + awm1;
   const float aw2 = float(p.aw * p.aw);
   const float db0 = px * px + py * py + aw2 * (awm1 * (2.0f + awm1));
-  const float ez = p.particle_ez ? float(p.particle_ez[ip])
-                                 : (p.slice_eloss ? float(p.slice_eloss[slice]) : 0.0f);
+  const float ez = p.particle_ez ? float(ld_global(&p.particle_ez[ip]))
+                                 : (p.slice_eloss ? float(ld_global(&p.slice_eloss[slice])) : 0.0f);
   float rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
+  {
 #pragma unroll
-  for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
-    if (NF == 0 && i >= nf) break;
-    gather<float>(p.f[i], slice, x, y, awloc, rr[i], ri[i]);
+    for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
+      if (NF == 0 && i >= nf) break;
+      gather_sum<float>(p.f[i], q[i], awloc, rr[i], ri[i]);
+    }
   }
   float r0[NF > 0 ? NF : kMaxFields];
 #pragma unroll
@@ -817,12 +868,14 @@ struct SplitScratch {
   int atomic_win;
 };
 
-template <class P, int NF, bool DEP, int NH>
+// pf issues the loads of the caller's next particle, inside the push right
+// after the field cells' loads (also when the push is off).
+template <class P, int NF, bool DEP, int NH, class PF>
 __device__ __forceinline__ void process(bool diag_on, const ParticleParams& prm, const MixedK& mk, int s,
                                         int64_t i, BeamPtr<P> b, P x, P y, P px, P py, P g,
                                         double th, P scl, typename V2<P>::T* src, double2* win,
                                         int wx0, int wy0, int ww, int wh, int* s_fb,
-                                        PassState<P, NH>& ps) {
+                                        PassState<P, NH>& ps, PF&& pf) {
   const int st = prm.stages;
   const double g0 = prm.gamma0;
   const int ng = prm.dep.ngrid;
@@ -832,18 +885,18 @@ __device__ __forceinline__ void process(bool diag_on, const ParticleParams& prm,
   double base_t = 0.0, base_s = 0.0, base_c = 1.0;
   float base_r = 0.0f;  // MIXED: reduced base_t
   bool have_base = false;
-  if (st & ST_PUSH) {
+  {
+    const bool on = (st & ST_PUSH) != 0;  // pf runs inside either way
     if constexpr (sizeof(P) == 8) {
       double gg = g;
-      push64<NF>(prm.push, s, i, x, y, px, py, gg, th, base_t, base_s, base_c);
+      push64<NF>(prm.push, s, i, x, y, px, py, gg, th, base_t, base_s, base_c, on, pf);
       g = gg;
-      have_base = prm.push.f[0].rharm == 1.0;
     } else {
       float gg = g;
-      push32<NF>(prm.push, mk, s, i, x, y, px, py, gg, th, base_t, base_r);
+      push32<NF>(prm.push, mk, s, i, x, y, px, py, gg, th, base_t, base_r, on, pf);
       g = gg;
-      have_base = prm.push.f[0].rharm == 1.0;
     }
+    have_base = on && prm.push.f[0].rharm == 1.0;
   }
   if (st & ST_R56) th += prm.r56 * gshift(g, g0);
   if (st & ST_T2) transverse<P>(prm.t2, g0, g, x, y, px, py, false);
@@ -1162,7 +1215,9 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
   }
 
   // Software-pipelined particle loop: the next particle's state is loaded
-  // while the current one is processed.
+  // while the current one is processed (issued after the current particle's
+  // field cells, whose wait would otherwise cover these loads too: memory
+  // counters retire in issue order).
   int64_t i = i0 + tid;
   bool have = i < i1;
   P nx = P(0), ny = P(0), npx = P(0), npy = P(0), ngm = P(0);
@@ -1180,16 +1235,19 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     const double th = nth;
     const int64_t inext = i + NT;
     const bool hnext = inext < i1;
-    if (hnext) {
-      nx = ld_stream(&b.x[inext]);
-      ny = ld_stream(&b.y[inext]);
-      npx = ld_stream(&b.px[inext]);
-      npy = ld_stream(&b.py[inext]);
-      ngm = ld_stream(&b.g[inext]);
-      nth = ld_stream(&b.th[inext]);
-    }
+    // unconditional (the last particle reloads itself): a load under a
+    // branch would make the counter wait for the cells conservative
+    const int64_t ipf = hnext ? inext : i;
+    auto prefetch = [&]() {
+      nx = ld_stream(&b.x[ipf]);
+      ny = ld_stream(&b.y[ipf]);
+      npx = ld_stream(&b.px[ipf]);
+      npy = ld_stream(&b.py[ipf]);
+      ngm = ld_stream(&b.g[ipf]);
+      nth = ld_stream(&b.th[ipf]);
+    };
     process<P, NF, DEP, NH>(diag_on, prm, mk, s, i, b, x, y, px, py, g, th, scl, src, win, wx0, wy0, ww,
-                            wh, s_fb, ps);
+                            wh, s_fb, ps, prefetch);
     i = inext;
     have = hnext;
   }
