@@ -513,8 +513,11 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       return o;
     };
     // The thread's column is fixed, so its x moments are dx and dx^2 times
-    // its sum of |u|^2, taken once after the loop.
-    double sw = 0.0;
+    // its sum of |u|^2, taken once after the loop.  MIXED sums its G / RPI
+    // cells in FP32 (as the particle pass sums its moments per thread); the
+    // sums across threads and tiles are FP64.
+    using A = std::conditional_t<sizeof(P) == 4, float, double>;
+    A sw = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0;
     fftr::static_for<0, G / RPI>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const int row = i * RPI + sc_row;
@@ -522,20 +525,24 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       const T2 o = out(row, gi);
       st_stream(&plane[gi], o);
       if (partials) {
-        const double re = o.x, im = o.y;
-        const double wei = re * re + im * im;
-        const double dy = row + shift;
+        const A re = o.x, im = o.y;
+        const A wei = re * re + im * im;
+        const A dy = static_cast<A>(row + shift);
         sw += wei;
-        acc[3] += dy * wei;
-        acc[4] += dy * dy * wei;
-        acc[5] += re;
-        acc[6] += im;
+        s3 += dy * wei;
+        s4 += dy * dy * wei;
+        s5 += re;
+        s6 += im;
       }
     });
     if (partials) {
       acc[0] = sw;
-      acc[1] = dx * sw;
-      acc[2] = dx * dx * sw;
+      acc[1] = dx * static_cast<double>(sw);
+      acc[2] = dx * dx * static_cast<double>(sw);
+      acc[3] = s3;
+      acc[4] = s4;
+      acc[5] = s5;
+      acc[6] = s6;
       // the centre cell, recomputed as stored by the thread that holds it
       const int crow = static_cast<int>(icenter / G), ccol = static_cast<int>(icenter % G);
       if (col == ccol && crow >= sc_row && (crow - sc_row) % RPI == 0) {
