@@ -638,15 +638,31 @@ __global__ void gather_source_kernel(const typename C2<P>::T* src, const int4* r
 // of O(N^2) per line).
 // ---------------------------------------------------------------------------
 
+// Launch shape of the convolution passes: as many waves (up to four) as fit
+// their transpose buffers in 72 KiB (48 KiB for MIXED lines of 1024 points,
+// where four waves measured slower than two), the padded lines taking twice
+// the LDS of N-point ones; TW columns per column-pass tile (a power of two).
+template <class P, int M>
+struct ConvCfg {
+  using S = fftr::Shape<M>;
+  static constexpr int kWaveBytes = S::LPW * S::BUF * 2 * static_cast<int>(sizeof(P));
+  static constexpr int kLds = sizeof(P) == 8 || M <= 512 ? 72 * 1024 : 48 * 1024;
+  static constexpr int kWaves = kWaveBytes * 4 <= kLds ? 4 : (kWaveBytes * 2 <= kLds ? 2 : 1);
+  static constexpr int kThreads = 64 * kWaves;
+  static constexpr int TW = kWaves * S::LPW < M ? kWaves * S::LPW : M;
+  static constexpr int kBuf = kWaves * S::LPW * S::BUF;
+  static constexpr int kColBuf = M * TW > kBuf ? M * TW : kBuf;  // covers N * TW (N < M)
+};
+
 // Row pass: rows of N points, each held on the lanes of a line slot as in
 // fft_rows_reg_kernel (without its prefetch: the padded lines already take
 // twice the registers).
 template <class P, int M>
-__global__ __launch_bounds__((RegCfg<P, M>::kThreads)) void fft_rows_conv_kernel(
+__global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_rows_conv_kernel(
     typename C2<P>::T* u, int64_t nrows, int N, const double2* tw_d, const double2* mult_d) {
   using T2 = typename C2<P>::T;
   using S = fftr::Shape<M>;
-  using C = RegCfg<P, M>;
+  using C = ConvCfg<P, M>;
   constexpr int R = S::R, L = S::L;
   __shared__ T2 tw[M];
   __shared__ T2 mu[M];
@@ -692,23 +708,16 @@ __global__ __launch_bounds__((RegCfg<P, M>::kThreads)) void fft_rows_conv_kernel
 // its tile (row segments per instruction, swizzled columns), transformed as
 // above, and written back with 2S added inside the slice's source rectangle
 // (or the filtered source of the work plane) and the field sums of the tile.
-template <class P, int M>
-struct ConvCfg {
-  using C = RegCfg<P, M>;
-  static constexpr int TW = C::kTile;  // power of two
-  static constexpr int kBuf = C::kWaves * fftr::Shape<M>::LPW * fftr::Shape<M>::BUF;
-  static constexpr int kColBuf = M * TW > kBuf ? M * TW : kBuf;  // covers N * TW (N < M)
-};
 
 template <class P, int M>
-__global__ __launch_bounds__((RegCfg<P, M>::kThreads)) void fft_cols_conv_kernel(
+__global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_cols_conv_kernel(
     typename C2<P>::T* u, int nslice, int N, const double2* tw_d, const double2* mult_d,
     const typename C2<P>::T* src, const int4* rect, int first, int add_src, const double2* work,
     double* partials) {
   using T2 = typename C2<P>::T;
   using S = fftr::Shape<M>;
-  using C = RegCfg<P, M>;
-  using K = ConvCfg<P, M>;
+  using C = ConvCfg<P, M>;
+  using K = C;
   constexpr int R = S::R, L = S::L, TW = K::TW;
   constexpr int NT = C::kThreads;
   __shared__ T2 tw[M];
@@ -878,7 +887,7 @@ void cols_reg(Field* f, typename C2<P>::T* data, int mode, double scale, int add
 
 template <class P, int M>
 void rows_conv(Field* f, typename C2<P>::T* data) {
-  using C = RegCfg<P, M>;
+  using C = ConvCfg<P, M>;
   constexpr int LPW = fftr::Shape<M>::LPW;
   const int N = f->ngrid;
   const int64_t nrows = static_cast<int64_t>(f->nslice) * N;
@@ -892,7 +901,7 @@ void rows_conv(Field* f, typename C2<P>::T* data) {
 template <class P, int M>
 void cols_conv(Field* f, typename C2<P>::T* data, int add_src, double* partials) {
   using T2 = typename C2<P>::T;
-  using C = RegCfg<P, M>;
+  using C = ConvCfg<P, M>;
   const int N = f->ngrid;
   const int ntile = (N + ConvCfg<P, M>::TW - 1) / ConvCfg<P, M>::TW;
   hipLaunchKernelGGL((fft_cols_conv_kernel<P, M>), dim3(f->nslice * ntile), dim3(C::kThreads), 0,
