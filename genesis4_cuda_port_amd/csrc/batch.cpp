@@ -91,6 +91,7 @@ struct g4h_batch {
   int nblocks = 0;
   std::vector<uint8_t> src_all;  // per step: the source is on in every group
   std::vector<BatchRunDesc> hdesc;  // host copy of druns
+  bool any_split = false;            // some run's particle pass takes K > 1 parts
   DevBuf<int2> dparts;              // (run, part) of each particle workgroup
   int nparts = 0;
   DevBuf<double> sred;              // split scratch of every run
@@ -251,11 +252,13 @@ struct g4h_batch {
     std::vector<int2> parts;
     std::vector<int> kof(hdesc.size());
     size_t ktot = 0;
+    any_split = false;
     for (size_t r = 0; r < hdesc.size(); ++r) {
       const auto [gi, si] = where[r];
       const Beam* bm = groups[static_cast<size_t>(gi)].beam;
       const int np = bm->off_h[static_cast<size_t>(si) + 1] - bm->off_h[static_cast<size_t>(si)];
       kof[r] = std::max(1, std::min(16, (np + 2047) / 2048));
+      any_split = any_split || kof[r] > 1;
       for (int q = 0; q < kof[r]; ++q) parts.push_back(make_int2(static_cast<int>(r), q));
       ktot += static_cast<size_t>(kof[r]);
     }
@@ -407,8 +410,8 @@ struct g4h_batch {
     for (int k = k0; k < k1; ++k) {
       const int r = rec_of_step[static_cast<size_t>(k)];
       const bool src = src_all[static_cast<size_t>(k)] != 0;
-      launch_batch_step_particles(ctx, prec, nharm, druns.get(), dparts.get(), nparts, k,
-                                  win_cells, r >= 0);
+      launch_batch_step_particles(ctx, prec, nharm, druns.get(), nruns, dparts.get(), nparts, k,
+                                  win_cells, r >= 0, any_split);
       launch_batch_adi_step(ctx, prec, dblocks.get(), nblocks, k, gmax, src, r >= 0, src);
       if (r >= 0) {
         launch_batch_field_records(ctx, druns.get(), nruns, r);

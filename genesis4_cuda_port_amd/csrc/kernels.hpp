@@ -185,10 +185,12 @@ struct BatchSegDesc {
 // d_parts: (run, part) of each workgroup, every run's K parts in order.  A
 // split run's parts deposit their out-of-window particles with atomics onto
 // its source plane, which launch_batch_adi_step clears behind its y sweep
-// (clear_src) so that each step's deposition starts from zero.
+// (clear_src) so that each step's deposition starts from zero.  With
+// any_split, a second launch (one workgroup per run of d_runs[nruns])
+// publishes the split runs' source rectangles, footprints and sums.
 void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
-                                 const BatchRunDesc* d_runs, const int2* d_parts, int nparts,
-                                 int k, int win_cells, bool sums);
+                                 const BatchRunDesc* d_runs, int nruns, const int2* d_parts,
+                                 int nparts, int k, int win_cells, bool sums, bool any_split);
 void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc* d_blocks,
                            int nblocks, int k, int gmax, bool add_src, bool stats,
                            bool clear_src);

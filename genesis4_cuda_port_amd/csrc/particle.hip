@@ -866,6 +866,11 @@ struct SplitScratch {
   // written region), both empty between launches; the last part only
   // publishes them (no K-window combine; win is unused)
   int atomic_win;
+  // nonzero (with atomic_win): no part counts arrivals or publishes; a
+  // following launch does (batch_publish_kernel).  Spares every part the
+  // device-scope release and acquire fences of the count, which write back
+  // and invalidate its L2.
+  int deferred;
 };
 
 // pf issues the loads of the caller's next particle, inside the push right
@@ -1340,6 +1345,7 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
         r[kSumMax + k] = v;
       }
     }
+    if (sp.atomic_win && sp.deferred) return;  // published by the next launch
     __threadfence();
     __syncthreads();
     __shared__ int s_last;
@@ -1746,8 +1752,50 @@ __global__ __launch_bounds__(kThreads) void batch_step_particle_kernel(
   const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
   BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
                static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
-  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1, 1};
+  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1, 1, 1};
   particle_slice<P, 1, true, NH, kThreads, true>(b, R.off, prm, sp, R.slice, R.K, pr.y, lds);
+}
+
+// After a per-step batch launch whose split runs defer publication: per
+// split run (one workgroup each), the slice's source rectangle and footprint
+// from the boxes its parts folded with atomics (then emptied for the next
+// step) and, with sums, the slice's beam sums from the K parts' partial sums
+// in part order: what the last part of a counting launch does.
+template <class P>
+__global__ __launch_bounds__(256) void batch_publish_kernel(const BatchRunDesc* __restrict__ runs,
+                                                            int k, int sums) {
+  const BatchRunDesc& R = runs[blockIdx.x];
+  const int K = R.K;
+  if (K <= 1) return;
+  const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
+  const int s = R.slice;
+  const int tid = threadIdx.x;
+  if ((prm.stages & ST_DEP) && tid == 0) {
+    const int4 t = R.sfb[0], r = R.sfb[1];
+    prm.dep.rect[s] = r.x > r.z ? make_int4(0, 0, 0, 0) : r;
+    prm.dep.bbox[s] = t;
+    R.sfb[0] = R.sfb[1] = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
+  }
+  if (!sums || !(prm.stages & ST_DIAG)) return;
+  const int nharm = prm.diag.nharm;
+  const int nsum = 12 + 2 * nharm;
+  const double* parts = R.sred;
+  double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
+  if (tid < nsum) {
+    double v = 0.0;
+    for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
+    out[tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12)] = v;
+  } else if (tid == 64) {
+    out[G4H_B_COUNT] = static_cast<double>(R.off[s + 1] - R.off[s]);
+  } else if (tid >= 128 && tid < 128 + kNMinMax) {
+    const int m = tid - 128;
+    double v = parts[kSumMax + m];
+    for (int q = 1; q < K; ++q)
+      v = (m & 1) ? fmax(v, parts[q * kRedStride + kSumMax + m])
+                  : fmin(v, parts[q * kRedStride + kSumMax + m]);
+    if (sizeof(P) == 4 && m >= 8) v += prm.gamma0;  // MIXED energy extremes were shifted
+    out[G4H_B_XMIN + m] = v;
+  }
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int n, int v) {
@@ -1879,8 +1927,8 @@ void launch_batch_persistent(Context* ctx, g4h_precision prec, int nharm,
 size_t batch_split_red_stride() { return kRedStride; }
 
 void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
-                                 const BatchRunDesc* d_runs, const int2* d_parts, int nparts,
-                                 int k, int win_cells, bool sums) {
+                                 const BatchRunDesc* d_runs, int nruns, const int2* d_parts,
+                                 int nparts, int k, int win_cells, bool sums, bool any_split) {
   if (nparts <= 0) return;
   require(nharm >= 1 && nharm <= kMaxHarm, "nharm must be in [1, 8]");
   const size_t lds = sizeof(double2) * static_cast<size_t>(win_cells);
@@ -1905,6 +1953,15 @@ void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
     else
       go(batch_step_particle_kernel<float, 8>);
   }
+  if (!any_split) return;
+  ctx->launch(G4H_K_OTHER, [&] {
+    if (prec == G4H_FP64)
+      hipLaunchKernelGGL(batch_publish_kernel<double>, dim3(nruns), dim3(256), 0, ctx->stream,
+                         d_runs, k, sums ? 1 : 0);
+    else
+      hipLaunchKernelGGL(batch_publish_kernel<float>, dim3(nruns), dim3(256), 0, ctx->stream,
+                         d_runs, k, sums ? 1 : 0);
+  });
 }
 
 // ---- beam sums without allocation ----------------------------------------
