@@ -96,7 +96,6 @@ struct g4h_batch {
   int nparts = 0;
   DevBuf<double> sred;              // split scratch of every run
   DevBuf<int4> sfb;
-  DevBuf<int32_t> scnt;
   DevBuf<BatchRunDesc> druns;
   DevBuf<int32_t> drec;
   std::vector<DevBuf<const double2*>> dadi;  // per group: factors of each step
@@ -108,7 +107,6 @@ struct g4h_batch {
     dparts.release();
     sred.release();
     sfb.release();
-    scnt.release();
     druns.release();
     drec.release();
     dadi.clear();
@@ -267,8 +265,6 @@ struct g4h_batch {
     const size_t red = batch_split_red_stride();
     sred.alloc(c, ktot * red);
     sfb.alloc(c, ktot);
-    scnt.alloc(c, hdesc.size());
-    G4_HIP(hipMemsetAsync(scnt.get(), 0, sizeof(int32_t) * hdesc.size(), c->stream));
     const std::vector<int4> empty(ktot, make_int4(0x7fffffff, 0x7fffffff, -1, -1));
     G4_HIP(hipMemcpyAsync(sfb.get(), empty.data(), sizeof(int4) * ktot, hipMemcpyHostToDevice,
                           c->stream));
@@ -279,7 +275,6 @@ struct g4h_batch {
       d.swin = nullptr;
       d.sred = sred.get() + at * red;
       d.sfb = sfb.get() + at;
-      d.scnt = scnt.get() + r;
       at += static_cast<size_t>(kof[r]);
     }
     G4_HIP(hipMemcpyAsync(druns.get(), hdesc.data(), sizeof(BatchRunDesc) * hdesc.size(),

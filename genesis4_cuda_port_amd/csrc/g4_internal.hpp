@@ -285,7 +285,6 @@ struct Beam {
   DevBuf<double2> split_win;
   DevBuf<double> split_red;
   DevBuf<int4> split_fb;
-  DevBuf<int32_t> split_cnt;  // zero between passes
   int max_slice_np = 0;
   // short-range space charge (g4h_beam_init_efield); ef.nz == 0: off
   g4h_efield ef{};

@@ -157,7 +157,6 @@ struct BatchRunDesc {
   double2* swin;               // [K][window cells]
   double* sred;                // [K][sums]
   int4* sfb;                   // [K]
-  int32_t* scnt;               // arrival counter (zero between launches)
   int slice;                   // the run's slice (= plane) in its group
   int nrun;                    // runs in the group
   int G;

@@ -852,26 +852,63 @@ struct PassState {
 
 // Scratch of a slice split over K workgroups (gridDim.y = K > 1): each
 // part's deposition window, its fallback-cell box and its diagnostic sums,
-// and a per-slice arrival counter (zero between launches).
+// combined by a following launch.
 constexpr int kSumMax = 12 + 2 * kMaxHarm;
 constexpr int kRedStride = kSumMax + kNMinMax;
 struct SplitScratch {
   double2* win;  // [nslice][K][window_cap]
   double* red;   // [nslice][K][kRedStride]
   int4* fb;      // [nslice][K]
-  int32_t* cnt;  // [nslice]
-  int own;       // nonzero: the arrays are this slice's own ([K][..], one counter)
+  int own;       // nonzero: the arrays are this slice's own ([K][..])
   // nonzero: each part adds its window to the (cleared) source plane with
   // atomics and folds its boxes into fb[0] (non-zero cells) and fb[1] (the
-  // written region), both empty between launches; the last part only
-  // publishes them (no K-window combine; win is unused)
+  // written region), both empty between launches (no K-window combine; win
+  // is unused)
   int atomic_win;
-  // nonzero (with atomic_win): no part counts arrivals or publishes; a
-  // following launch does (batch_publish_kernel).  Spares every part the
-  // device-scope release and acquire fences of the count, which write back
-  // and invalidate its L2.
-  int deferred;
 };
+
+// The deposition window of a slice: the cells it touched last step (bb)
+// plus a 2-cell margin, clipped to the grid and to cap cells (centred);
+// empty without a footprint or without current.
+__device__ __forceinline__ void place_window(int4 bb, bool on, int ng, int cap, int& wx0,
+                                             int& wy0, int& ww, int& wh) {
+  wx0 = wy0 = ww = wh = 0;
+  if (!(bb.x <= bb.z && on)) return;
+  int x0 = max(0, bb.x - 2), y0 = max(0, bb.y - 2);
+  const int x1 = min(ng, bb.z + 3), y1 = min(ng, bb.w + 3);
+  int w = x1 - x0, h = y1 - y0;
+  if (w * h > cap) {
+    int side = 1;
+    while ((side + 1) * (side + 1) <= cap) ++side;
+    const int nw = min(w, max(side, cap / h));
+    const int nh = min(h, cap / nw);
+    x0 += (w - nw) / 2;
+    y0 += (h - nh) / 2;
+    w = nw;
+    h = nh;
+  }
+  wx0 = x0;
+  wy0 = y0;
+  ww = w;
+  wh = h;
+}
+
+// A slice's source rectangle (the written window and the cells reached
+// through the global fallback) and footprint (its non-zero cells, bb, and
+// those fallback cells, fb) for the field step and the next step's window.
+__device__ __forceinline__ void publish_boxes(const ParticleParams& prm, int s, int wx0, int wy0,
+                                              int ww, int wh, const int* bb, const int* fb) {
+  int4 r = ww > 0 && wh > 0 ? make_int4(wx0, wy0, wx0 + ww, wy0 + wh)
+                            : make_int4(0x7fffffff, 0x7fffffff, -1, -1);
+  int4 t = make_int4(bb[0], bb[1], bb[2], bb[3]);
+  if (fb[0] <= fb[2]) {
+    r = make_int4(min(r.x, fb[0]), min(r.y, fb[1]), max(r.z, fb[2] + 1), max(r.w, fb[3] + 1));
+    t = make_int4(min(t.x, fb[0]), min(t.y, fb[1]), max(t.z, fb[2]), max(t.w, fb[3]));
+  }
+  if (r.x > r.z) r = make_int4(0, 0, 0, 0);
+  prm.dep.rect[s] = r;
+  prm.dep.bbox[s] = t;
+}
 
 // pf issues the loads of the caller's next particle, inside the push right
 // after the field cells' loads (also when the push is off).
@@ -1150,26 +1187,7 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
       bb = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
       __syncthreads();
     }
-    if (bb.x <= bb.z && scl != P(0)) {
-      int x0 = max(0, bb.x - 2), y0 = max(0, bb.y - 2);
-      int x1 = min(ng, bb.z + 3), y1 = min(ng, bb.w + 3);
-      int w = x1 - x0, h = y1 - y0;
-      const int cap = prm.dep.window_cap;
-      if (w * h > cap) {
-        int side = 1;
-        while ((side + 1) * (side + 1) <= cap) ++side;
-        int nw = min(w, max(side, cap / h));
-        int nh = min(h, cap / nw);
-        x0 += (w - nw) / 2;
-        y0 += (h - nh) / 2;
-        w = nw;
-        h = nh;
-      }
-      wx0 = x0;
-      wy0 = y0;
-      ww = w;
-      wh = h;
-    }
+    place_window(bb, scl != P(0), ng, prm.dep.window_cap, wx0, wy0, ww, wh);
     // zero what the previous deposition left in this slice's source plane
     // (a split slice has that done by a pass before this one)
     const int4 old = prm.dep.rect[s];
@@ -1278,8 +1296,11 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     }
   }
   if (K > 1) {
-    // this part's window, fallback box and sums to scratch; the last part
-    // to arrive goes on to combine all K of them in part order
+    // this part's window (or its atomic adds), fallback box and sums to
+    // scratch; a following launch combines the K parts (split_combine_kernel,
+    // batch_publish_kernel), so that no part waits on another: an arrival
+    // count would need device-scope release and acquire fences, which write
+    // back and invalidate the L2 of every part's CU
     __syncthreads();
     const int64_t ss = sp.own ? 0 : s;  // the slice's slot in the scratch arrays
     const int64_t part = ss * K + kp;
@@ -1345,52 +1366,116 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
         r[kSumMax + k] = v;
       }
     }
-    if (sp.atomic_win && sp.deferred) return;  // published by the next launch
-    __threadfence();
-    __syncthreads();
-    __shared__ int s_last;
-    if (tid == 0) s_last = atomicAdd(&sp.cnt[ss], 1) == K - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (tid == 0) {
-      if constexpr (DEP) if (dep_on) {
-        if (sp.atomic_win) {
-          const int4 t = sp.fb[ss * K], r = sp.fb[ss * K + 1];
-          prm.dep.rect[s] = r.x > r.z ? make_int4(0, 0, 0, 0) : r;
-          prm.dep.bbox[s] = t;
-          sp.fb[ss * K] = sp.fb[ss * K + 1] = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
-        } else {
-          for (int q = 0; q < K; ++q) {
-            const int4 f = sp.fb[ss * K + q];
-            s_fb[0] = min(s_fb[0], f.x);
-            s_fb[1] = min(s_fb[1], f.y);
-            s_fb[2] = max(s_fb[2], f.z);
-            s_fb[3] = max(s_fb[3], f.w);
-          }
-        }
-      }
-      sp.cnt[ss] = 0;
-    }
-    __syncthreads();
+    return;
   }
 
-  if constexpr (DEP) if (dep_on && !(K > 1 && sp.atomic_win)) {
+  if constexpr (DEP) if (dep_on) {
     __syncthreads();
     // write the window with plain stores; its non-zero cells give the
     // footprint for the next step's window
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
-    const double2* parts = sp.win + (sp.own ? 0 : static_cast<int64_t>(s)) * K * prm.dep.window_cap;
     for (int k = tid; k < ww * wh; k += NT) {
       const int cy = wy0 + k / ww, cx = wx0 + k % ww;
-      double2 v = win[k];
-      if (K > 1) {
-        v = parts[k];
-        for (int q = 1; q < K; ++q) {
-          const double2 a = parts[static_cast<int64_t>(q) * prm.dep.window_cap + k];
-          v.x += a.x;
-          v.y += a.y;
-        }
+      const double2 v = win[k];
+      T2 o;
+      o.x = static_cast<P>(v.x);
+      o.y = static_cast<P>(v.y);
+      src[cx + static_cast<int64_t>(cy) * ng] = o;
+      if (v.x != 0.0 || v.y != 0.0) {
+        mnx = min(mnx, cx);
+        mny = min(mny, cy);
+        mxx = max(mxx, cx);
+        mxy = max(mxy, cy);
+      }
+    }
+    atomicMin(&s_bb[0], mnx);
+    atomicMin(&s_bb[1], mny);
+    atomicMax(&s_bb[2], mxx);
+    atomicMax(&s_bb[3], mxy);
+    __syncthreads();
+    if (tid == 0) publish_boxes(prm, s, wx0, wy0, ww, wh, s_bb, s_fb);
+  }
+
+  if constexpr (NH > 0) if (diag_on) {
+    __syncthreads();
+    double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
+    // raw layout: 12 sums (G4H_B_G1..G4H_B_YPY), the count, 10 min/max, then
+    // (cos, sin) sums per harmonic
+    if (tid < nsum) {
+      double v = 0.0;
+      for (int w = 0; w < NW; ++w) v += s_red[w][tid];
+      out[tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12)] = v;
+    } else if (tid == 64) {
+      out[G4H_B_COUNT] = static_cast<double>(i1s - i0s);
+    } else if (tid >= 128 && tid < 128 + kNMinMax) {
+      const int k = tid - 128;
+      double v = s_red[0][NSUM + k];
+      for (int w = 1; w < NW; ++w)
+        v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
+      if (sizeof(P) == 4 && k >= 8) v += g0;  // MIXED energy extremes were shifted
+      // order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
+      out[G4H_B_XMIN + k] = v;
+    }
+  }
+}
+
+template <class P, int NF, bool DEP, int NH, bool TAB>
+__global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
+                                                            ParticleParams prm_arg,
+                                                            const ParticleParams* __restrict__ tab,
+                                                            SplitScratch sp) {
+  extern __shared__ double2 win[];
+  particle_slice<P, NF, DEP, NH, kThreads>(b, off, slice_params<TAB, P>(prm_arg, tab), sp,
+                                           blockIdx.x, gridDim.y, blockIdx.y, win);
+}
+
+// The combine of a split particle pass (K parts per slice; one workgroup per
+// slice, after the pass): the slice's window as the sum of the K parts'
+// windows in part order, written to the source plane, its rectangle and
+// footprint, and the beam sums of the K parts in part order.  Parts that
+// added their windows with atomics (the batch's per-step launch) publish
+// through batch_publish_kernel instead.
+template <class P, bool TAB>
+__global__ __launch_bounds__(kThreads) void split_combine_kernel(
+    const int32_t* off, ParticleParams prm_arg, const ParticleParams* __restrict__ tab,
+    SplitScratch sp, int K) {
+  using T2 = typename V2<P>::T;
+  const ParticleParams& prm = slice_params<TAB, P>(prm_arg, tab);
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t ss = sp.own ? 0 : s;
+  __shared__ int s_bb[4];
+  __shared__ int s_fb[4];
+  if ((prm.stages & ST_DEP) && !sp.atomic_win) {
+    const int ng = prm.dep.ngrid;
+    const int cap = prm.dep.window_cap;
+    T2* src = static_cast<T2*>(prm.dep.src) + static_cast<int64_t>(s) * ng * ng;
+    const P scl = static_cast<P>(prm.dep.cur_over_npar[s] * prm.dep.scl_factor);
+    int wx0, wy0, ww, wh;
+    place_window(prm.dep.bbox[s], scl != P(0), ng, cap, wx0, wy0, ww, wh);
+    if (tid == 0) {
+      s_bb[0] = s_bb[1] = 0x7fffffff;
+      s_bb[2] = s_bb[3] = -1;
+      s_fb[0] = s_fb[1] = 0x7fffffff;
+      s_fb[2] = s_fb[3] = -1;
+      for (int q = 0; q < K; ++q) {
+        const int4 f = sp.fb[ss * K + q];
+        s_fb[0] = min(s_fb[0], f.x);
+        s_fb[1] = min(s_fb[1], f.y);
+        s_fb[2] = max(s_fb[2], f.z);
+        s_fb[3] = max(s_fb[3], f.w);
+      }
+    }
+    __syncthreads();
+    const double2* parts = sp.win + ss * K * cap;
+    int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+    for (int k = tid; k < ww * wh; k += kThreads) {
+      const int cy = wy0 + k / ww, cx = wx0 + k % ww;
+      double2 v = parts[k];
+      for (int q = 1; q < K; ++q) {
+        const double2 a = parts[static_cast<int64_t>(q) * cap + k];
+        v.x += a.x;
+        v.y += a.y;
       }
       T2 o;
       o.x = static_cast<P>(v.x);
@@ -1408,75 +1493,28 @@ __device__ __forceinline__ void particle_slice(BeamPtr<P> b, const int32_t* off,
     atomicMax(&s_bb[2], mxx);
     atomicMax(&s_bb[3], mxy);
     __syncthreads();
-    if (tid == 0) {
-      int4 r;
-      if (ww > 0 && wh > 0) {
-        r = make_int4(wx0, wy0, wx0 + ww, wy0 + wh);
-      } else {
-        r = make_int4(0x7fffffff, 0x7fffffff, -1, -1);
-      }
-      int4 t = make_int4(s_bb[0], s_bb[1], s_bb[2], s_bb[3]);
-      if (s_fb[0] <= s_fb[2]) {
-        r.x = min(r.x, s_fb[0]);
-        r.y = min(r.y, s_fb[1]);
-        r.z = max(r.z, s_fb[2] + 1);
-        r.w = max(r.w, s_fb[3] + 1);
-        t.x = min(t.x, s_fb[0]);
-        t.y = min(t.y, s_fb[1]);
-        t.z = max(t.z, s_fb[2]);
-        t.w = max(t.w, s_fb[3]);
-      }
-      if (r.x > r.z) r = make_int4(0, 0, 0, 0);
-      prm.dep.rect[s] = r;
-      prm.dep.bbox[s] = t;
-    }
+    if (tid == 0) publish_boxes(prm, s, wx0, wy0, ww, wh, s_bb, s_fb);
   }
-
-  if constexpr (NH > 0) if (diag_on) {
-    __syncthreads();
-    const double* parts = sp.red + (sp.own ? 0 : static_cast<int64_t>(s)) * K * kRedStride;
-    double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
-    // raw layout: 12 sums (G4H_B_G1..G4H_B_YPY), the count, 10 min/max, then
-    // (cos, sin) sums per harmonic
-    if (tid < nsum) {
-      double v = 0.0;
-      if (K == 1) {
-        for (int w = 0; w < NW; ++w) v += s_red[w][tid];
-      } else {
-        for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
-      }
-      const int dst = tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12);
-      out[dst] = v;
-    } else if (tid == 64) {
-      out[G4H_B_COUNT] = static_cast<double>(i1s - i0s);
-    } else if (tid >= 128 && tid < 128 + kNMinMax) {
-      const int k = tid - 128;
-      double v;
-      if (K == 1) {
-        v = s_red[0][NSUM + k];
-        for (int w = 1; w < NW; ++w)
-          v = (k & 1) ? fmax(v, s_red[w][NSUM + k]) : fmin(v, s_red[w][NSUM + k]);
-      } else {
-        v = parts[kSumMax + k];
-        for (int q = 1; q < K; ++q)
-          v = (k & 1) ? fmax(v, parts[q * kRedStride + kSumMax + k])
-                      : fmin(v, parts[q * kRedStride + kSumMax + k]);
-      }
-      if (sizeof(P) == 4 && k >= 8) v += g0;  // MIXED energy extremes were shifted
-      // order: xmin,xmax,pxmin,pxmax,ymin,ymax,pymin,pymax,gmin,gmax
-      out[G4H_B_XMIN + k] = v;
-    }
+  if (!(prm.stages & ST_DIAG)) return;
+  const int nharm = prm.diag.nharm;
+  const int nsum = 12 + 2 * nharm;
+  const double* parts = sp.red + ss * K * kRedStride;
+  double* out = prm.diag.raw + static_cast<int64_t>(s) * G4H_BEAM_RAW(nharm);
+  if (tid < nsum) {
+    double v = 0.0;
+    for (int q = 0; q < K; ++q) v += parts[q * kRedStride + tid];
+    out[tid < 12 ? tid : G4H_BEAM_STATS + (tid - 12)] = v;
+  } else if (tid == 64) {
+    out[G4H_B_COUNT] = static_cast<double>(off[s + 1] - off[s]);
+  } else if (tid >= 128 && tid < 128 + kNMinMax) {
+    const int m = tid - 128;
+    double v = parts[kSumMax + m];
+    for (int q = 1; q < K; ++q)
+      v = (m & 1) ? fmax(v, parts[q * kRedStride + kSumMax + m])
+                  : fmin(v, parts[q * kRedStride + kSumMax + m]);
+    if (sizeof(P) == 4 && m >= 8) v += prm.gamma0;  // MIXED energy extremes were shifted
+    out[G4H_B_XMIN + m] = v;
   }
-}
-
-template <class P, int NF, bool DEP, int NH, bool TAB>
-__global__ __launch_bounds__(kThreads) void particle_kernel(BeamPtr<P> b, const int32_t* off,
-                                                            ParticleParams prm_arg,
-                                                            const ParticleParams* __restrict__ tab,
-                                                            SplitScratch sp) {
-  extern __shared__ double2 win[];
-  particle_slice<P, NF, DEP, NH, kThreads>(b, off, slice_params<TAB, P>(prm_arg, tab), sp,
-                                           blockIdx.x, gridDim.y, blockIdx.y, win);
 }
 
 template <class P>
@@ -1584,19 +1622,14 @@ void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
   const int K = split_of(ctx, b);
   if (!tab && sizeof(P) == 8)
     tab = static_cast<const ParticleParams*>(ctx->param_push(&p, sizeof(p)));
-  SplitScratch sp{nullptr, nullptr, nullptr, nullptr};
+  SplitScratch sp{nullptr, nullptr, nullptr, 0, 0};
   if (K > 1) {
     const size_t parts = static_cast<size_t>(b->nslice) * K;
     const size_t nwin = DEP ? parts * static_cast<size_t>(p.dep.window_cap) : 0;
     if (b->split_win.size() < nwin) b->split_win.alloc(ctx, nwin);
     if (b->split_red.size() < parts * kRedStride) b->split_red.alloc(ctx, parts * kRedStride);
     if (b->split_fb.size() < parts) b->split_fb.alloc(ctx, parts);
-    if (b->split_cnt.size() < static_cast<size_t>(b->nslice)) {
-      b->split_cnt.alloc(ctx, static_cast<size_t>(b->nslice));
-      G4_HIP(hipMemsetAsync(b->split_cnt.get(), 0, sizeof(int32_t) * b->nslice, ctx->stream));
-    }
-    sp = SplitScratch{b->split_win.get(), b->split_red.get(), b->split_fb.get(),
-                      b->split_cnt.get()};
+    sp = SplitScratch{b->split_win.get(), b->split_red.get(), b->split_fb.get(), 0, 0};
     if (DEP) {
       ctx->launch(G4H_K_OTHER, [&] {
         hipLaunchKernelGGL((split_prepass_kernel<P, TAB>), dim3(b->nslice), dim3(256), 0,
@@ -1608,6 +1641,12 @@ void launch_t(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P> bp,
     hipLaunchKernelGGL((particle_kernel<P, NF, DEP, NH, TAB>), dim3(b->nslice, K),
                        dim3(kThreads), lds, ctx->stream, bp, b->off.get(), p, tab, sp);
   });
+  if (K > 1 && (DEP || NH > 0)) {
+    ctx->launch(G4H_K_OTHER, [&] {
+      hipLaunchKernelGGL((split_combine_kernel<P, TAB>), dim3(b->nslice), dim3(kThreads), 0,
+                         ctx->stream, b->off.get(), p, tab, sp, K);
+    });
+  }
   ctx->param_commit();
 }
 
@@ -1659,7 +1698,7 @@ __device__ __forceinline__ void batch_particles(const BatchRunDesc& R, const Par
                                                 double2* win) {
   BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
                static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
-  const SplitScratch none{nullptr, nullptr, nullptr, nullptr};
+  const SplitScratch none{nullptr, nullptr, nullptr, 0, 0};
   // one copy of the pass; deposition and sums as the step's stages ask
   particle_slice<P, 1, true, NH, kBatchThreads, true>(b, R.off, prm, none, R.slice, 1, 0, win);
 }
@@ -1752,7 +1791,7 @@ __global__ __launch_bounds__(kThreads) void batch_step_particle_kernel(
   const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
   BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
                static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
-  const SplitScratch sp{R.swin, R.sred, R.sfb, R.scnt, 1, 1, 1};
+  const SplitScratch sp{R.swin, R.sred, R.sfb, 1, 1};
   particle_slice<P, 1, true, NH, kThreads, true>(b, R.off, prm, sp, R.slice, R.K, pr.y, lds);
 }
 
@@ -1798,11 +1837,6 @@ __global__ __launch_bounds__(256) void batch_publish_kernel(const BatchRunDesc* 
   }
 }
 
-__global__ void fill_i32_kernel(int32_t* p, int n, int v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 // Beam-sums pass (ST_DIAG only) whose scratch the caller provides: d_prm is
 // the FP64 parameter block in device memory (unused in MIXED, which takes
 // it by value), sp the split pass's partials and counters for K > 1.
@@ -1813,6 +1847,12 @@ void launch_diag_given(Context* ctx, Beam* b, const ParticleParams& p, BeamPtr<P
     hipLaunchKernelGGL((particle_kernel<P, 1, false, NH, false>), dim3(b->nslice, K),
                        dim3(kThreads), 0, ctx->stream, bp, b->off.get(), p, d_prm, sp);
   });
+  if (K > 1) {
+    ctx->launch(G4H_K_OTHER, [&] {
+      hipLaunchKernelGGL((split_combine_kernel<P, false>), dim3(b->nslice), dim3(kThreads), 0,
+                         ctx->stream, b->off.get(), p, d_prm, sp, K);
+    });
+  }
 }
 
 template <class P>
@@ -2006,7 +2046,6 @@ size_t diag_beam_noalloc_bytes(Context* ctx, const Beam* b, int nharm, int* spli
     const size_t parts = static_cast<size_t>(b->nslice) * K;
     n += align256(parts * kRedStride * sizeof(double));
     n += align256(parts * sizeof(int4));
-    n += align256(static_cast<size_t>(b->nslice) * sizeof(int32_t));
   }
   return n;
 }
@@ -2033,19 +2072,13 @@ void diag_beam_raw_noalloc(Context* ctx, Beam* b, int nharm, double* raw) {
   auto* d_raw = reinterpret_cast<double*>(base + at);
   const size_t nraw = static_cast<size_t>(b->nslice) * G4H_BEAM_RAW(nharm);
   at += align256(nraw * sizeof(double));
-  SplitScratch sp{nullptr, nullptr, nullptr, nullptr};
+  SplitScratch sp{nullptr, nullptr, nullptr, 0, 0};
   if (K > 1) {
     const size_t parts = static_cast<size_t>(b->nslice) * K;
     sp.red = reinterpret_cast<double*>(base + at);
     at += align256(parts * kRedStride * sizeof(double));
     sp.fb = reinterpret_cast<int4*>(base + at);
     at += align256(parts * sizeof(int4));
-    sp.cnt = reinterpret_cast<int32_t*>(base + at);
-    // the arrival counters start at zero (a fill kernel, not a memset node)
-    const int n = b->nslice;
-    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, sp.cnt,
-                       n, 0);
-    G4_HIP(hipGetLastError());
   }
   ParticleParams p{};
   p.gamma0 = b->gamma_ref;
