@@ -91,6 +91,9 @@ struct g4h_batch {
   int nblocks = 0;
   std::vector<uint8_t> src_all;  // per step: the source is on in every group
   std::vector<BatchRunDesc> hdesc;  // host copy of druns
+  // every run's parameter block per step, [step][run] (the per-step
+  // particle launch reads its runs' blocks through one kernel argument)
+  DevBuf<ParticleParams> steptab;
   bool any_split = false;            // some run's particle pass takes K > 1 parts
   DevBuf<int2> dparts;              // (run, part) of each particle workgroup
   int nparts = 0;
@@ -109,6 +112,7 @@ struct g4h_batch {
     sfb.release();
     druns.release();
     drec.release();
+    steptab.release();
     dadi.clear();
     dcs.clear();
     if (pdone) (void)hipEventDestroy(pdone);
@@ -243,6 +247,17 @@ struct g4h_batch {
       src_all[static_cast<size_t>(k)] = a;
     }
     build_run_descs(c);
+    {
+      const size_t nr = hdesc.size();
+      steptab.alloc(c, static_cast<size_t>(ns) * nr);
+      for (size_t r = 0; r < nr; ++r) {
+        const BatchRunDesc& d = hdesc[r];
+        G4_HIP(hipMemcpy2DAsync(steptab.get() + r, sizeof(ParticleParams) * nr, d.tab + d.slice,
+                                sizeof(ParticleParams) * static_cast<size_t>(d.nrun),
+                                sizeof(ParticleParams), static_cast<size_t>(ns),
+                                hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
     // Particle workgroups: a run of n particles takes K = ceil(n / 2048)
     // parts (at most 16), eight iterations of the 256-thread loop each, so
     // that a batch of runs of a few thousand particles fills the GPU.  Each
@@ -405,8 +420,9 @@ struct g4h_batch {
     for (int k = k0; k < k1; ++k) {
       const int r = rec_of_step[static_cast<size_t>(k)];
       const bool src = src_all[static_cast<size_t>(k)] != 0;
-      launch_batch_step_particles(ctx, prec, nharm, druns.get(), nruns, dparts.get(), nparts, k,
-                                  win_cells, r >= 0, any_split);
+      launch_batch_step_particles(ctx, prec, nharm, druns.get(), nruns, dparts.get(), nparts,
+                                  steptab.get() + static_cast<size_t>(k) * nruns, k, win_cells,
+                                  r >= 0, any_split);
       launch_batch_adi_step(ctx, prec, dblocks.get(), nblocks, k, gmax, src, r >= 0, src);
       if (r >= 0) {
         launch_batch_field_records(ctx, druns.get(), nruns, r);

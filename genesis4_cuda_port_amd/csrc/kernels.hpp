@@ -13,38 +13,52 @@ namespace g4 {
 // with nontemporal accesses on MI355X.
 typedef float nt_f2 __attribute__((ext_vector_type(2)));
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
+// The streaming accesses go through global-address-space pointers: a pointer
+// that the compiler cannot trace to a kernel argument (one read from a run
+// descriptor in device memory) would otherwise give flat accesses, which
+// also count on the LDS / scalar-load counter, so that every LDS wait would
+// wait for them too.
+#define G4_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ float ld_stream(const float* p) {
+  return __builtin_nontemporal_load((G4_GLOBAL const float*)(p));
+}
+__device__ __forceinline__ double ld_stream(const double* p) {
+  return __builtin_nontemporal_load((G4_GLOBAL const double*)(p));
+}
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
-  const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(p));
+  const nt_f2 v = __builtin_nontemporal_load((G4_GLOBAL const nt_f2*)(p));
   return make_float2(v.x, v.y);
 }
 __device__ __forceinline__ double2 ld_stream(const double2* p) {
-  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  const nt_d2 v = __builtin_nontemporal_load((G4_GLOBAL const nt_d2*)(p));
   return make_double2(v.x, v.y);
 }
-__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(float* p, float v) {
+  __builtin_nontemporal_store(v, (G4_GLOBAL float*)(p));
+}
+__device__ __forceinline__ void st_stream(double* p, double v) {
+  __builtin_nontemporal_store(v, (G4_GLOBAL double*)(p));
+}
 __device__ __forceinline__ void st_stream(float2* p, float2 v) {
   nt_f2 t;
   t.x = v.x;
   t.y = v.y;
-  __builtin_nontemporal_store(t, reinterpret_cast<nt_f2*>(p));
+  __builtin_nontemporal_store(t, (G4_GLOBAL nt_f2*)(p));
 }
 __device__ __forceinline__ void st_stream(double2* p, double2 v) {
   nt_d2 t;
   t.x = v.x;
   t.y = v.y;
-  __builtin_nontemporal_store(t, reinterpret_cast<nt_d2*>(p));
+  __builtin_nontemporal_store(t, (G4_GLOBAL nt_d2*)(p));
 }
 
 // Loads through pointers that address global memory but that the compiler
 // cannot prove to (read from a parameter block in device memory): as flat
 // loads they would also count on the counter of LDS and scalar loads, so
 // every wait for a scalar load would wait for them too.
-typedef __attribute__((address_space(1))) const nt_f2* gl_f2;
-typedef __attribute__((address_space(1))) const nt_d2* gl_d2;
-typedef __attribute__((address_space(1))) const double* gl_d;
+typedef G4_GLOBAL const nt_f2* gl_f2;
+typedef G4_GLOBAL const nt_d2* gl_d2;
+typedef G4_GLOBAL const double* gl_d;
 __device__ __forceinline__ float2 ld_global(const float2* p) {
   const nt_f2 v = *(gl_f2)(p);
   return make_float2(v.x, v.y);
@@ -184,12 +198,14 @@ struct BatchSegDesc {
 // d_parts: (run, part) of each workgroup, every run's K parts in order.  A
 // split run's parts deposit their out-of-window particles with atomics onto
 // its source plane, which launch_batch_adi_step clears behind its y sweep
-// (clear_src) so that each step's deposition starts from zero.  With
-// any_split, a second launch (one workgroup per run of d_runs[nruns])
-// publishes the split runs' source rectangles, footprints and sums.
+// (clear_src) so that each step's deposition starts from zero.  d_tabk: the
+// runs' parameter blocks of step k, [nruns].  With any_split, a second
+// launch (one workgroup per run of d_runs[nruns]) publishes the split runs'
+// source rectangles, footprints and sums.
 void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
                                  const BatchRunDesc* d_runs, int nruns, const int2* d_parts,
-                                 int nparts, int k, int win_cells, bool sums, bool any_split);
+                                 int nparts, const ParticleParams* d_tabk, int k, int win_cells,
+                                 bool sums, bool any_split);
 void launch_batch_adi_step(Context* ctx, g4h_precision prec, const BatchSegDesc* d_blocks,
                            int nblocks, int k, int gmax, bool add_src, bool stats,
                            bool clear_src);

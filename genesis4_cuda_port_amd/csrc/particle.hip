@@ -597,7 +597,12 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
   // the caller's loads in flight.  pf runs here whether or not the push is
   // on (one call site: the loaded registers need no copies).
   Gathered<double> q[NF > 0 ? NF : kMaxFields];
+  double ez = 0.0;
   if (on) {
+    // load_longitudinal_ez (BeamSolverCUDA.cpp:183-191), before the cells so
+    // that its wait is theirs
+    ez = p.particle_ez ? ld_global(&p.particle_ez[ip])
+                       : (p.slice_eloss ? ld_global(&p.slice_eloss[slice]) : 0.0);
 #pragma unroll
     for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
       if (NF == 0 && i >= nf) break;
@@ -613,9 +618,6 @@ __device__ __forceinline__ void push64(const PushParams& p, int slice, int64_t i
   const double dx = x - p.ax, dy = y - p.ay;
   const double awloc = 1.0 + 0.5 * (p.kx * dx * dx + p.ky * dy * dy) + p.gradx * dx + p.grady * dy;
   const double btpar = 1.0 + px * px + py * py + p.aw * p.aw * awloc * awloc;
-  // load_longitudinal_ez (BeamSolverCUDA.cpp:183-191)
-  const double ez = p.particle_ez ? ld_global(&p.particle_ez[ip])
-                                  : (p.slice_eloss ? ld_global(&p.slice_eloss[slice]) : 0.0);
   double rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
   {
 #pragma unroll
@@ -677,8 +679,11 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
                                        double& theta, double& base_t, float& base_r, bool on,
                                        PF&& pf) {
   const int nf = NF > 0 ? NF : p.nfield;
-  Gathered<float> q[NF > 0 ? NF : kMaxFields];  // cells, then pf, as in push64
+  Gathered<float> q[NF > 0 ? NF : kMaxFields];  // ez and cells, then pf, as in push64
+  float ez = 0.0f;
   if (on) {
+    ez = p.particle_ez ? float(ld_global(&p.particle_ez[ip]))
+                       : (p.slice_eloss ? float(ld_global(&p.slice_eloss[slice])) : 0.0f);
 #pragma unroll
     for (int i = 0; i < (NF > 0 ? NF : kMaxFields); ++i) {
       if (NF == 0 && i >= nf) break;
@@ -697,8 +702,6 @@ __device__ __forceinline__ void push32(const PushParams& p, const MixedK& m, int
   const float awloc = 1.0f + awm1;
   const float aw2 = float(p.aw * p.aw);
   const float db0 = px * px + py * py + aw2 * (awm1 * (2.0f + awm1));
-  const float ez = p.particle_ez ? float(ld_global(&p.particle_ez[ip]))
-                                 : (p.slice_eloss ? float(ld_global(&p.slice_eloss[slice])) : 0.0f);
   float rr[NF > 0 ? NF : kMaxFields], ri[NF > 0 ? NF : kMaxFields];
   {
 #pragma unroll
@@ -752,8 +755,15 @@ __device__ __forceinline__ double gshift(float dg, double) { return double(dg); 
 __device__ __forceinline__ void lds_add(double* a, double v) {
   __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ void glob_add(double* a, double v) { unsafeAtomicAdd(a, v); }
-__device__ __forceinline__ void glob_add(float* a, float v) { unsafeAtomicAdd(a, v); }
+// Source-plane atomics as global (not flat) operations: a pending flat
+// operation forces every later wait of the wave to drain all its memory
+// counters, the next particle's loads included.
+__device__ __forceinline__ void glob_add(double* a, double v) {
+  __hip_atomic_fetch_add((G4_GLOBAL double*)(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void glob_add(float* a, float v) {
+  __hip_atomic_fetch_add((G4_GLOBAL float*)(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -1784,11 +1794,16 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
 // workgroups per CU.
 template <class P, int NH>
 __global__ __launch_bounds__(kThreads) void batch_step_particle_kernel(
-    const BatchRunDesc* __restrict__ runs, const int2* __restrict__ parts, int k) {
+    const BatchRunDesc* __restrict__ runs, const int2* __restrict__ parts,
+    const ParticleParams* __restrict__ tabk) {
   extern __shared__ double2 lds[];
   const int2 pr = parts[blockIdx.x];  // (run, part)
   const BatchRunDesc& R = runs[pr.x];
-  const ParticleParams& prm = R.tab[static_cast<int64_t>(k) * R.nrun + R.slice];
+  // through a restrict kernel argument the parameters are scalar loads; read
+  // through the run descriptor's table pointer they would be vector loads
+  // inside the particle loop (the compiler cannot show that the loop's
+  // stores leave that table alone), each waited for with everything else
+  const ParticleParams& prm = tabk[pr.x];
   BeamPtr<P> b{static_cast<P*>(R.x),  static_cast<P*>(R.y), static_cast<P*>(R.px),
                static_cast<P*>(R.py), static_cast<P*>(R.g), R.th};
   const SplitScratch sp{R.swin, R.sred, R.sfb, 1, 1};
@@ -1968,14 +1983,15 @@ size_t batch_split_red_stride() { return kRedStride; }
 
 void launch_batch_step_particles(Context* ctx, g4h_precision prec, int nharm,
                                  const BatchRunDesc* d_runs, int nruns, const int2* d_parts,
-                                 int nparts, int k, int win_cells, bool sums, bool any_split) {
+                                 int nparts, const ParticleParams* d_tabk, int k, int win_cells,
+                                 bool sums, bool any_split) {
   if (nparts <= 0) return;
   require(nharm >= 1 && nharm <= kMaxHarm, "nharm must be in [1, 8]");
   const size_t lds = sizeof(double2) * static_cast<size_t>(win_cells);
   auto go = [&](auto kern) {
     ctx->launch(G4H_K_PARTICLE, [&] {
       hipLaunchKernelGGL(kern, dim3(nparts), dim3(kThreads), lds, ctx->stream, d_runs, d_parts,
-                         k);
+                         d_tabk);
     });
   };
   if (prec == G4H_FP64) {

@@ -66,8 +66,10 @@ uint64_t estimate_bytes(const g4h_request& q, int nsteps) {
   for (int i = 0; i < nsteps; ++i) nout += q.steps[i].outstep ? 1 : 0;
   const uint64_t hist =
       static_cast<uint64_t>(1 + nout) * q.nslice * (G4H_BEAM_RAW(q.nharm) + G4H_FIELD_RAW) * 8;
-  // a steady-state request in a batch also holds one parameter block per step
-  const uint64_t tables = q.kind == 0 ? static_cast<uint64_t>(nsteps) * sizeof(g4::ParticleParams) : 0;
+  // a steady-state request in a batch also holds two parameter blocks per
+  // step (its group's table and the batch's per-step table of all runs)
+  const uint64_t tables =
+      q.kind == 0 ? 2 * static_cast<uint64_t>(nsteps) * sizeof(g4::ParticleParams) : 0;
   return np * sp + cells * (2 * sc + (q.solver == G4H_SOLVER_ADI ? 16 + sc : 0)) + hist + tables +
          (1u << 20);
 }
