@@ -254,7 +254,7 @@ g4h_status g4h_diag_field_raw(g4h_context* ctx, g4h_field* field, double* field_
  * forbid hipMalloc).  No counterpart in the reference.  Same results as
  * g4h_diag_beam_raw, bit for bit.  Its device memory (the FP64 parameter
  * block, the per-slice sums before the copy to the host, the split pass's
- * per-part partials and arrival counters) is one fixed 8 MiB array per GPU,
+ * per-part partials and boxes) is one fixed 8 MiB array per GPU,
  * held until the process ends: nslice * (23 + 2*nharm) doubles plus, for
  * beams of few slices, ~60 doubles per workgroup of the split pass must fit,
  * about 40 000 slices at nharm = 1; larger beams get G4H_ERR_INVALID_ARGUMENT
