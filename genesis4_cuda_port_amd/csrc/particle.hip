@@ -442,12 +442,11 @@ __device__ __forceinline__ void gather_sum(const GatherField& f, const Gathered<
 // step well inside the FEL bandwidth) the series stop at d^9 and d^8: the
 // first omitted terms, d^11/11! < 2e-21 and d^10/10! < 3e-19, are as far
 // below FP64 rounding.  Larger offsets take the full routine.
-constexpr bool kShortNear = true;
 __device__ __forceinline__ void sincos_near(double d, double a, double s0, double c0, double& s,
                                             double& c) {
   const double d2 = d * d;
   double sd, cd;
-  if (kShortNear && __builtin_amdgcn_ballot_w64(!(fabs(d) <= 0.0625)) == 0) {
+  if (__builtin_amdgcn_ballot_w64(!(fabs(d) <= 0.0625)) == 0) {
     double ps = sconst(1.0 / 362880.0);  // 1/9!
     ps = fma(ps, d2, sconst(-1.0 / 5040.0));
     ps = fma(ps, d2, sconst(1.0 / 120.0));
