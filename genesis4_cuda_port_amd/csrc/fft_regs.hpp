@@ -212,9 +212,23 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Ordering between the writes and reads of a line's transpose buffer: the
+// line's lanes are in one wave, or (BLOCK) spread over the workgroup's waves.
+template <bool BLOCK>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (BLOCK) {
+    __syncthreads();
+  } else {
+    wave_lds_sync();
+  }
+}
+
 // Forward transform: natural layout in, spectrum in pi order out.
-// tw[k2*L + t] = W_G^(t*k2); buf is this line's transpose buffer.
-template <class T2, int G>
+// tw[k2*L + t] = W_G^(t*k2); buf is this line's transpose buffer, whose
+// element e sits at buf[e * STRIDE] (STRIDE > 1: the buffers of STRIDE lines
+// interleaved, so that lanes of different lines at one e hit adjacent LDS
+// words); BLOCK: the line's lanes are in different waves of the workgroup.
+template <class T2, int G, int STRIDE = 1, bool BLOCK = false>
 __device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* tw, T2* buf) {
   using S = Shape<G>;
   constexpr int R = S::R, L = S::L;
@@ -226,14 +240,14 @@ __device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* t
     });
     static_for<0, R>([&](auto ic) {
       constexpr int k2 = decltype(ic)::value;
-      buf[k2 * S::LP + t] = v[k2];
+      buf[(k2 * S::LP + t) * STRIDE] = v[k2];
     });
-    wave_lds_sync();
+    line_sync<BLOCK>();
     static_for<0, R>([&](auto ic) {
       constexpr int q = decltype(ic)::value;  // q = j*L + n1
-      v[q] = buf[(t + L * (q / L)) * S::LP + (q % L)];
+      v[q] = buf[((t + L * (q / L)) * S::LP + (q % L)) * STRIDE];
     });
-    wave_lds_sync();
+    line_sync<BLOCK>();
     static_for<0, S::SUB>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       dft_regs<T2, R, L, j * L, false>(v);
@@ -242,7 +256,7 @@ __device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* t
 }
 
 // Inverse (unnormalised): spectrum in pi order in, natural layout out.
-template <class T2, int G>
+template <class T2, int G, int STRIDE = 1, bool BLOCK = false>
 __device__ __forceinline__ void inverse(T2 (&v)[Shape<G>::R], int t, const T2* tw, T2* buf) {
   using S = Shape<G>;
   constexpr int R = S::R, L = S::L;
@@ -253,14 +267,16 @@ __device__ __forceinline__ void inverse(T2 (&v)[Shape<G>::R], int t, const T2* t
     });
     static_for<0, R>([&](auto ic) {
       constexpr int q = decltype(ic)::value;
-      buf[(t + L * (q / L)) * S::LP + (q % L)] = v[q];
+      buf[((t + L * (q / L)) * S::LP + (q % L)) * STRIDE] = v[q];
     });
-    wave_lds_sync();
+    line_sync<BLOCK>();
     static_for<0, R>([&](auto ic) {
       constexpr int k2 = decltype(ic)::value;
-      v[k2] = buf[k2 * S::LP + t];
+      v[k2] = buf[(k2 * S::LP + t) * STRIDE];
     });
-    wave_lds_sync();
+    // a wave's next line reuses its buffer at once; a workgroup's callers
+    // synchronise before they reuse theirs
+    if constexpr (!BLOCK) wave_lds_sync();
     static_for<1, R>([&](auto ic) {
       constexpr int k2 = decltype(ic)::value;
       v[k2] = cmulc(v[k2], tw[k2 * L + t]);

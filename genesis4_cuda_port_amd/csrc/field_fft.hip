@@ -8,8 +8,9 @@
 //   u' = Cy( Cx(u) ) + 2S,   Cx = IFFT_x Px FFT_x / G  (rows),
 //                            Cy = IFFT_y Py FFT_y / G  (columns).
 // The row pass holds whole rows in the registers of a wave and does
-// FFT -> Px -> IFFT there; the column pass stages a tile of columns through
-// LDS (coalesced row segments), transforms each column in registers, adds 2S
+// FFT -> Px -> IFFT there; the column pass loads a tile of columns as
+// coalesced row segments, transforms each column in registers (its lanes in
+// several waves, transposed through LDS), adds 2S
 // inside the slice's source rectangle and, on output steps, reduces the field
 // diagnostics of the tile.  Each plane crosses HBM twice per step (read+write
 // per pass).
@@ -304,15 +305,17 @@ struct RegCfg {
   static constexpr int kThreads = 64 * kWaves;
   // column tile: one column per line slot of the workgroup
   static constexpr int kTile = kWaves * S::LPW < G ? kWaves * S::LPW : G;
-  // the column pass stages its tile (G rows of kTile columns, swizzled)
-  // through LDS when every line slot holds a column, a tile row is at least
-  // 64 bytes, kTile is a power of two and the staged copy covers the G rows
-  // in whole instructions
-  static constexpr bool kStage = kWaves * S::LPW <= G && kThreads % kTile == 0 &&
-                                 G % (kThreads / kTile) == 0 && (kTile & (kTile - 1)) == 0 &&
-                                 kTile * 2 * static_cast<int>(sizeof(P)) >= 64;
+  // Row-segment tile of the column pass, when every line slot holds a
+  // column (kTile = kThreads / L) and a tile row is at least 64 bytes:
+  // thread (t = tid / kTile, column tid % kTile) loads and stores the rows
+  // t + L r of its column, so each load or store of a wave covers
+  // 64 / kTile whole row segments of the tile, and the registers are already
+  // in the natural layout of the line transforms.  A column's L lanes then sit in different
+  // waves: its transposes go through LDS buffers interleaved by column (the
+  // lanes of one row segment hit adjacent words) with workgroup barriers.
+  static constexpr bool kDirect = S::L > 1 && kWaves * S::LPW <= G &&
+                                  kTile * 2 * static_cast<int>(sizeof(P)) >= 64;
   static constexpr int kBuf = kWaves * S::LPW * S::BUF;
-  static constexpr int kColBuf = kStage && G * kTile > kBuf ? G * kTile : kBuf;
 };
 
 template <class T2>
@@ -389,11 +392,11 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
   }
 }
 
-// Column pass: a workgroup owns kTile adjacent columns of one ring plane;
-// the tile is staged through LDS with whole row segments per instruction
-// (or, for tiles narrower than 64 bytes, each line slot loads its column
-// directly), each wave transforms its columns in registers, and the tile is
-// stored back with the source added and the field sums of the tile reduced.
+// Column pass: a workgroup owns kTile adjacent columns of one ring plane,
+// loaded as whole row segments (RegCfg::kDirect; for tiles narrower than
+// 64 bytes each wave's line slots load their columns), transformed in
+// registers and stored back with the source added and the field sums of the
+// tile reduced.
 template <class P, int G>
 __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
     typename C2<P>::T* u, int nslice, const double2* tw_d, const double2* mult_d,
@@ -406,48 +409,32 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   constexpr int ntile = G / C::kTile;
   __shared__ T2 tw[G];
   __shared__ T2 mu[G];
-  __shared__ T2 buf[C::kColBuf];
+  __shared__ T2 buf[C::kBuf];
   __shared__ double red[C::kWaves][8];
-  load_table(tw, tw_d, G);
-  if (mode == COL_PROP) load_table(mu, mult_d, G);
-  __syncthreads();
   const int p = blockIdx.x / ntile;
   const int tile = blockIdx.x - p * ntile;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane / L, t = lane % L;
-  const int cslot = wave * S::LPW + g;
+  constexpr bool kDirect = C::kDirect;
+  constexpr int TW = C::kTile;
+  // line slot (column of the tile) and lane of the line
+  const int t = kDirect ? static_cast<int>(threadIdx.x) / TW : lane % L;
+  const int cslot = kDirect ? static_cast<int>(threadIdx.x) % TW : wave * S::LPW + lane / L;
   const int c = tile * C::kTile + cslot;  // column of this line slot
   const bool valid = cslot < C::kTile;
-  T2* lbuf = buf + cslot * S::BUF;
+  T2* lbuf = kDirect ? buf + cslot : buf + cslot * S::BUF;
+  constexpr int kStride = kDirect ? TW : 1;
   T2* plane = u + static_cast<int64_t>(p) * G * G;
-  // Staged tile: the workgroup's kTile columns of all G rows pass through
-  // LDS (reusing the transpose buffer), so each load and store instruction
-  // covers whole row segments of the tile instead of L rows x LPW columns.
-  // Cell (row, col) of the tile sits at row * TW + (col ^ (row % TW)): the
-  // row-segment copies write and read whole rows (any order within a row is
-  // conflict-free), and the line slots' reads of one column down L rows hit
-  // TW different positions, where a plain or +1-padded row would put
-  // several lanes on one LDS bank.
-  constexpr int TW = C::kTile;
-  constexpr bool kStage = C::kStage;
-  auto swz = [](int row, int col) { return row * TW + (col ^ (row & (TW - 1))); };
-  constexpr int RPI = C::kThreads / TW;  // rows per instruction of the staged copy
-  const int c0 = tile * TW;
   T2 v[R];
-  if constexpr (kStage) {
-    const int sc_col = threadIdx.x % TW, sc_row = threadIdx.x / TW;
-    fftr::static_for<0, G / RPI>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      const int row = i * RPI + sc_row;
-      buf[swz(row, sc_col)] = ld_stream(&plane[static_cast<int64_t>(row) * G + c0 + sc_col]);
-    });
-    __syncthreads();
+  if constexpr (kDirect) {
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
-      v[r] = buf[swz(t + L * r, cslot)];
+      v[r] = ld_stream(&plane[static_cast<int64_t>(t + L * r) * G + c]);
     });
-    __syncthreads();
-  } else {
+  }
+  load_table(tw, tw_d, G);
+  if (mode == COL_PROP) load_table(mu, mult_d, G);
+  __syncthreads();
+  if constexpr (!kDirect) {
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int r = decltype(ic)::value;
       v[r].x = 0;
@@ -455,7 +442,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       if (valid) v[r] = plane[static_cast<int64_t>(t + L * r) * G + c];
     });
   }
-  fftr::forward<T2, G>(v, t, tw, lbuf);
+  fftr::forward<T2, G, kStride, kDirect>(v, t, tw, lbuf);
   if (mode == COL_PROP) {
     fftr::static_for<0, R>([&](auto ic) {
       constexpr int q = decltype(ic)::value;
@@ -473,7 +460,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       v[q].y *= f;
     });
   }
-  fftr::inverse<T2, G>(v, t, tw, lbuf);
+  fftr::inverse<T2, G, kStride, kDirect>(v, t, tw, lbuf);
 
   const P sc = static_cast<P>(scale);
   const int s = ((p - first) % nslice + nslice) % nslice;  // beam slice of this ring plane
@@ -485,23 +472,13 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   const int64_t icenter = (static_cast<int64_t>(G) * G - 1) / 2;
   const bool src_col = sp && c >= rc.x && c < rc.z;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  if constexpr (kStage) {
-    __syncthreads();  // the inverse transform's LDS use is over
-    fftr::static_for<0, R>([&](auto ic) {
-      constexpr int r = decltype(ic)::value;
-      buf[swz(t + L * r, cslot)] = v[r];
-    });
-    __syncthreads();
-    const int sc_col = threadIdx.x % TW, sc_row = threadIdx.x / TW;
-    const int col = c0 + sc_col;
-    const bool scol = sp && col >= rc.x && col < rc.z;
-    const double dx = col + shift;
-    // the stored value of tile cell (row, this thread's column)
-    auto out = [&](int row, int64_t gi) {
-      T2 o = buf[swz(row, sc_col)];
+  if constexpr (kDirect) {
+    const double dx = c + shift;
+    // the stored value of cell (row, this thread's column)
+    auto out = [&](int row, int64_t gi, T2 o) {
       o.x *= sc;
       o.y *= sc;
-      if (scol && row >= rc.y && row < rc.w) {
+      if (src_col && row >= rc.y && row < rc.w) {
         const T2 sv = ld_stream(&sp[gi]);
         o.x += P(2) * sv.x;
         o.y += P(2) * sv.y;
@@ -513,16 +490,16 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       return o;
     };
     // The thread's column is fixed, so its x moments are dx and dx^2 times
-    // its sum of |u|^2, taken once after the loop.  MIXED sums its G / RPI
+    // its sum of |u|^2, taken once after the loop.  MIXED sums its R
     // cells in FP32 (as the particle pass sums its moments per thread); the
     // sums across threads and tiles are FP64.
     using A = std::conditional_t<sizeof(P) == 4, float, double>;
     A sw = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0;
-    fftr::static_for<0, G / RPI>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      const int row = i * RPI + sc_row;
-      const int64_t gi = static_cast<int64_t>(row) * G + col;
-      const T2 o = out(row, gi);
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      const int row = t + L * r;
+      const int64_t gi = static_cast<int64_t>(row) * G + c;
+      const T2 o = out(row, gi, v[r]);
       st_stream(&plane[gi], o);
       if (partials) {
         const A re = o.x, im = o.y;
@@ -544,9 +521,9 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
       acc[5] = s5;
       acc[6] = s6;
       // the centre cell, recomputed as stored by the thread that holds it
-      const int crow = static_cast<int>(icenter / G), ccol = static_cast<int>(icenter % G);
-      if (col == ccol && crow >= sc_row && (crow - sc_row) % RPI == 0) {
-        const T2 o = out(crow, icenter);
+      constexpr int crow = (G * G - 1) / 2 / G, ccol = (G * G - 1) / 2 % G;
+      if (c == ccol && t == crow % L) {
+        const T2 o = out(crow, icenter, v[crow / L]);
         partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = o.x;
         partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = o.y;
       }
@@ -704,9 +681,10 @@ __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_rows_conv_kerne
 }
 
 // Column pass: a workgroup owns TW adjacent columns of one ring plane (the
-// last tile clipped at N), staged through LDS as fft_cols_reg_kernel stages
-// its tile (row segments per instruction, swizzled columns), transformed as
-// above, and written back with 2S added inside the slice's source rectangle
+// last tile clipped at N), staged through LDS (row segments per instruction;
+// cell (row, col) at row * TW + (col ^ (row % TW)), so that the line slots'
+// reads down one column hit different banks), transformed as above, and
+// written back with 2S added inside the slice's source rectangle
 // (or the filtered source of the work plane) and the field sums of the tile.
 
 template <class P, int M>
