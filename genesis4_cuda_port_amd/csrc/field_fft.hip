@@ -626,9 +626,9 @@ struct ConvCfg {
   static constexpr int kLds = sizeof(P) == 8 || M <= 512 ? 72 * 1024 : 48 * 1024;
   static constexpr int kWaves = kWaveBytes * 4 <= kLds ? 4 : (kWaveBytes * 2 <= kLds ? 2 : 1);
   static constexpr int kThreads = 64 * kWaves;
-  static constexpr int TW = kWaves * S::LPW < M ? kWaves * S::LPW : M;
+  // columns per column-pass tile: one line slot (L lanes) per column
+  static constexpr int TW = kThreads / S::L;
   static constexpr int kBuf = kWaves * S::LPW * S::BUF;
-  static constexpr int kColBuf = M * TW > kBuf ? M * TW : kBuf;  // covers N * TW (N < M)
 };
 
 // Row pass: rows of N points, each held on the lanes of a line slot as in
@@ -681,11 +681,12 @@ __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_rows_conv_kerne
 }
 
 // Column pass: a workgroup owns TW adjacent columns of one ring plane (the
-// last tile clipped at N), staged through LDS (row segments per instruction;
-// cell (row, col) at row * TW + (col ^ (row % TW)), so that the line slots'
-// reads down one column hit different banks), transformed as above, and
-// written back with 2S added inside the slice's source rectangle
-// (or the filtered source of the work plane) and the field sums of the tile.
+// last tile clipped at N), loaded and stored as row segments in the line
+// transforms' own layout as in fft_cols_reg_kernel (thread (t, column
+// tid % TW) holds rows t + L r, a column's lanes in several waves, the
+// transposes through column-interleaved LDS), transformed as above, and
+// written back with 2S added inside the slice's source rectangle (or the
+// filtered source of the work plane) and the field sums of the tile.
 
 template <class P, int M>
 __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_cols_conv_kernel(
@@ -695,59 +696,39 @@ __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_cols_conv_kerne
   using T2 = typename C2<P>::T;
   using S = fftr::Shape<M>;
   using C = ConvCfg<P, M>;
-  using K = C;
-  constexpr int R = S::R, L = S::L, TW = K::TW;
-  constexpr int NT = C::kThreads;
+  constexpr int R = S::R, L = S::L, TW = C::TW;
+  static_assert(C::kThreads / TW == L, "one line slot per column, L lanes each");
   __shared__ T2 tw[M];
   __shared__ T2 mu[M];
-  __shared__ T2 buf[K::kColBuf];
+  __shared__ T2 buf[C::kBuf];
   __shared__ double red[C::kWaves][8];
-  load_table(tw, tw_d, M);
-  load_table(mu, mult_d, M);
   const int ntile = (N + TW - 1) / TW;
   const int p = blockIdx.x / ntile;
   const int tile = blockIdx.x - p * ntile;
-  const int c0 = tile * TW;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane / L, t = lane % L;
-  const int cslot = wave * S::LPW + g;  // line slot; slots >= TW transform zeros
-  T2* plane = u + static_cast<int64_t>(p) * N * N;
-  auto swz = [](int row, int col) { return row * TW + (col ^ (row & (TW - 1))); };
-  const int sc_col = threadIdx.x % TW;
-  const int col = c0 + sc_col;
+  const int t = static_cast<int>(threadIdx.x) / TW;
+  const int cslot = static_cast<int>(threadIdx.x) % TW;
+  const int col = tile * TW + cslot;
   const bool colok = col < N;
-#pragma unroll 4
-  for (int row = threadIdx.x / TW; row < N; row += NT / TW) {
-    T2 z;
-    z.x = 0;
-    z.y = 0;
-    if (colok) z = ld_stream(&plane[static_cast<int64_t>(row) * N + col]);
-    buf[swz(row, sc_col)] = z;
-  }
-  __syncthreads();
+  T2* plane = u + static_cast<int64_t>(p) * N * N;
   T2 v[R];
   fftr::static_for<0, R>([&](auto ic) {
     constexpr int r = decltype(ic)::value;
     const int j = t + L * r;
     v[r].x = 0;
     v[r].y = 0;
-    if (cslot < TW && j < N) v[r] = buf[swz(j, cslot)];
+    if (colok && j < N) v[r] = ld_stream(&plane[static_cast<int64_t>(j) * N + col]);
   });
+  load_table(tw, tw_d, M);
+  load_table(mu, mult_d, M);
   __syncthreads();
-  T2* lbuf = buf + cslot * S::BUF;
-  fftr::forward<T2, M>(v, t, tw, lbuf);
+  T2* lbuf = buf + cslot;
+  fftr::forward<T2, M, TW, true>(v, t, tw, lbuf);
   fftr::static_for<0, R>([&](auto ic) {
     constexpr int q = decltype(ic)::value;
     v[q] = fftr::cmul(v[q], mu[q * L + t]);
   });
-  fftr::inverse<T2, M>(v, t, tw, lbuf);
-  __syncthreads();
-  fftr::static_for<0, R>([&](auto ic) {
-    constexpr int r = decltype(ic)::value;
-    const int j = t + L * r - (N - 1);
-    if (cslot < TW && j >= 0 && j < N) buf[swz(j, cslot)] = v[r];
-  });
-  __syncthreads();
+  fftr::inverse<T2, M, TW, true>(v, t, tw, lbuf);
 
   const int s = ((p - first) % nslice + nslice) % nslice;  // beam slice of this ring plane
   int4 rc = make_int4(0, 0, 0, 0);
@@ -760,34 +741,39 @@ __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_cols_conv_kerne
   const double dx = col + shift;
   double sw = 0.0, acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (colok) {
-    for (int row = threadIdx.x / TW; row < N; row += NT / TW) {
-      const int64_t gi = static_cast<int64_t>(row) * N + col;
-      T2 o = buf[swz(row, sc_col)];
-      if (scol && row >= rc.y && row < rc.w) {
-        const T2 sv = ld_stream(&sp[gi]);
-        o.x += P(2) * sv.x;
-        o.y += P(2) * sv.y;
-      } else if (wp) {
-        const double2 wv = ld_stream(&wp[gi]);
-        o.x += static_cast<P>(wv.x);
-        o.y += static_cast<P>(wv.y);
-      }
-      st_stream(&plane[gi], o);
-      if (partials) {
-        const double re = o.x, im = o.y;
-        const double wei = re * re + im * im;
-        const double dy = row + shift;
-        sw += wei;
-        acc[3] += dy * wei;
-        acc[4] += dy * dy * wei;
-        acc[5] += re;
-        acc[6] += im;
-        if (gi == icenter) {
-          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
-          partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+    // register r holds output row t + L r - (N - 1)
+    fftr::static_for<0, R>([&](auto ic) {
+      constexpr int r = decltype(ic)::value;
+      const int row = t + L * r - (N - 1);
+      if (row >= 0 && row < N) {
+        const int64_t gi = static_cast<int64_t>(row) * N + col;
+        T2 o = v[r];
+        if (scol && row >= rc.y && row < rc.w) {
+          const T2 sv = ld_stream(&sp[gi]);
+          o.x += P(2) * sv.x;
+          o.y += P(2) * sv.y;
+        } else if (wp) {
+          const double2 wv = ld_stream(&wp[gi]);
+          o.x += static_cast<P>(wv.x);
+          o.y += static_cast<P>(wv.y);
+        }
+        st_stream(&plane[gi], o);
+        if (partials) {
+          const double re = o.x, im = o.y;
+          const double wei = re * re + im * im;
+          const double dy = row + shift;
+          sw += wei;
+          acc[3] += dy * wei;
+          acc[4] += dy * dy * wei;
+          acc[5] += re;
+          acc[6] += im;
+          if (gi == icenter) {
+            partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p] = re;
+            partials[static_cast<int64_t>(nslice) * ntile * 8 + 2 * p + 1] = im;
+          }
         }
       }
-    }
+    });
   }
   if (partials) {
     acc[0] = sw;
