@@ -223,13 +223,36 @@ __device__ __forceinline__ void line_sync() {
   }
 }
 
+// One transpose of a line: v[k] goes to buf[wi(k) * STRIDE], then v[k] is read
+// back from buf[ri(k) * STRIDE].  A buffer of scalars (B = the component
+// type) takes the real parts, then the imaginary ones (half the LDS, two
+// more barriers); the reads of each pass overwrite only the component that
+// pass has written.
+template <class T2, int R, int STRIDE, bool BLOCK, class B, class WI, class RI>
+__device__ __forceinline__ void xpose(T2 (&v)[R], B* buf, WI&& wi, RI&& ri) {
+  if constexpr (!std::is_same_v<B, T2>) {
+    static_for<0, R>([&](auto ic) { buf[wi(ic) * STRIDE] = v[decltype(ic)::value].x; });
+    line_sync<BLOCK>();
+    static_for<0, R>([&](auto ic) { v[decltype(ic)::value].x = buf[ri(ic) * STRIDE]; });
+    line_sync<BLOCK>();
+    static_for<0, R>([&](auto ic) { buf[wi(ic) * STRIDE] = v[decltype(ic)::value].y; });
+    line_sync<BLOCK>();
+    static_for<0, R>([&](auto ic) { v[decltype(ic)::value].y = buf[ri(ic) * STRIDE]; });
+  } else {
+    static_for<0, R>([&](auto ic) { buf[wi(ic) * STRIDE] = v[decltype(ic)::value]; });
+    line_sync<BLOCK>();
+    static_for<0, R>([&](auto ic) { v[decltype(ic)::value] = buf[ri(ic) * STRIDE]; });
+  }
+}
+
 // Forward transform: natural layout in, spectrum in pi order out.
 // tw[k2*L + t] = W_G^(t*k2); buf is this line's transpose buffer, whose
 // element e sits at buf[e * STRIDE] (STRIDE > 1: the buffers of STRIDE lines
 // interleaved, so that lanes of different lines at one e hit adjacent LDS
-// words); BLOCK: the line's lanes are in different waves of the workgroup.
-template <class T2, int G, int STRIDE = 1, bool BLOCK = false>
-__device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* tw, T2* buf) {
+// words); BLOCK: the line's lanes are in different waves of the workgroup;
+// B: complex, or scalar for a transpose in two passes (xpose).
+template <class T2, int G, int STRIDE = 1, bool BLOCK = false, class B = T2>
+__device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* tw, B* buf) {
   using S = Shape<G>;
   constexpr int R = S::R, L = S::L;
   dft_regs<T2, R, R, 0, false>(v);
@@ -238,15 +261,14 @@ __device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* t
       constexpr int k2 = decltype(ic)::value;
       v[k2] = cmul(v[k2], tw[k2 * L + t]);
     });
-    static_for<0, R>([&](auto ic) {
-      constexpr int k2 = decltype(ic)::value;
-      buf[(k2 * S::LP + t) * STRIDE] = v[k2];
-    });
-    line_sync<BLOCK>();
-    static_for<0, R>([&](auto ic) {
-      constexpr int q = decltype(ic)::value;  // q = j*L + n1
-      v[q] = buf[((t + L * (q / L)) * S::LP + (q % L)) * STRIDE];
-    });
+    // v[k2] to row k2 of the buffer, register q = j*L + n1 from row
+    // t + L*j, column n1
+    xpose<T2, R, STRIDE, BLOCK>(
+        v, buf, [&](auto k2) { return decltype(k2)::value * S::LP + t; },
+        [&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          return (t + L * (q / L)) * S::LP + (q % L);
+        });
     line_sync<BLOCK>();
     static_for<0, S::SUB>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
@@ -256,8 +278,8 @@ __device__ __forceinline__ void forward(T2 (&v)[Shape<G>::R], int t, const T2* t
 }
 
 // Inverse (unnormalised): spectrum in pi order in, natural layout out.
-template <class T2, int G, int STRIDE = 1, bool BLOCK = false>
-__device__ __forceinline__ void inverse(T2 (&v)[Shape<G>::R], int t, const T2* tw, T2* buf) {
+template <class T2, int G, int STRIDE = 1, bool BLOCK = false, class B = T2>
+__device__ __forceinline__ void inverse(T2 (&v)[Shape<G>::R], int t, const T2* tw, B* buf) {
   using S = Shape<G>;
   constexpr int R = S::R, L = S::L;
   if constexpr (L > 1) {
@@ -265,15 +287,13 @@ __device__ __forceinline__ void inverse(T2 (&v)[Shape<G>::R], int t, const T2* t
       constexpr int j = decltype(jc)::value;
       dft_regs<T2, R, L, j * L, true>(v);
     });
-    static_for<0, R>([&](auto ic) {
-      constexpr int q = decltype(ic)::value;
-      buf[((t + L * (q / L)) * S::LP + (q % L)) * STRIDE] = v[q];
-    });
-    line_sync<BLOCK>();
-    static_for<0, R>([&](auto ic) {
-      constexpr int k2 = decltype(ic)::value;
-      v[k2] = buf[(k2 * S::LP + t) * STRIDE];
-    });
+    xpose<T2, R, STRIDE, BLOCK>(
+        v, buf,
+        [&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          return (t + L * (q / L)) * S::LP + (q % L);
+        },
+        [&](auto k2) { return decltype(k2)::value * S::LP + t; });
     // a wave's next line reuses its buffer at once; a workgroup's callers
     // synchronise before they reuse theirs
     if constexpr (!BLOCK) wave_lds_sync();

@@ -316,6 +316,11 @@ struct RegCfg {
   static constexpr bool kDirect = S::L > 1 && kWaves * S::LPW <= G &&
                                   kTile * 2 * static_cast<int>(sizeof(P)) >= 64;
   static constexpr int kBuf = kWaves * S::LPW * S::BUF;
+  // FP64 tiles at G = 128 and 256 transpose real and imaginary parts in
+  // turn through half the buffer (fftr::xpose), so that with at most 168
+  // registers a CU holds three workgroups instead of two.
+  static constexpr bool kSplit = kDirect && sizeof(P) == 8 && G >= 128 && G <= 256;
+  static constexpr int kMinWaves = kSplit ? 3 : 1;  // per SIMD
 };
 
 template <class T2>
@@ -398,7 +403,8 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_rows_reg_kernel(
 // registers and stored back with the source added and the field sums of the
 // tile reduced.
 template <class P, int G>
-__global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
+__global__ __launch_bounds__((RegCfg<P, G>::kThreads),
+                             (RegCfg<P, G>::kMinWaves)) void fft_cols_reg_kernel(
     typename C2<P>::T* u, int nslice, const double2* tw_d, const double2* mult_d,
     const double* sigmoid, int mode, double scale, const typename C2<P>::T* src, const int4* rect,
     int first, int add_src, const double2* work, double* partials) {
@@ -409,7 +415,9 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   constexpr int ntile = G / C::kTile;
   __shared__ T2 tw[G];
   __shared__ T2 mu[G];
-  __shared__ T2 buf[C::kBuf];
+  // line transpose buffers: complex, or scalars for the two-pass transposes
+  using B = std::conditional_t<C::kSplit, P, T2>;
+  __shared__ B buf[C::kBuf];
   __shared__ double red[C::kWaves][8];
   const int p = blockIdx.x / ntile;
   const int tile = blockIdx.x - p * ntile;
@@ -421,7 +429,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads)) void fft_cols_reg_kernel(
   const int cslot = kDirect ? static_cast<int>(threadIdx.x) % TW : wave * S::LPW + lane / L;
   const int c = tile * C::kTile + cslot;  // column of this line slot
   const bool valid = cslot < C::kTile;
-  T2* lbuf = kDirect ? buf + cslot : buf + cslot * S::BUF;
+  B* lbuf = kDirect ? buf + cslot : buf + cslot * S::BUF;
   constexpr int kStride = kDirect ? TW : 1;
   T2* plane = u + static_cast<int64_t>(p) * G * G;
   T2 v[R];
