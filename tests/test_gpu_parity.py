@@ -289,7 +289,7 @@ def test_fft_sizes(gpu_ctx, prec, ngrid):
     assert relerr(fraw, ofraw) < (1e-10 if prec == api.FP64 else 1e-4)
 
 
-@pytest.mark.parametrize("ngrid", [32, 128, 33])
+@pytest.mark.parametrize("ngrid", [32, 128, 256, 33])
 def test_fft_filter_fp64(gpu_ctx, ngrid):
     c = cases.make_case(nslice=3, npart=1024, ngrid=ngrid, seed_power=1e6)
     st = quad_step()
@@ -306,6 +306,30 @@ def test_fft_filter_fp64(gpu_ctx, ngrid):
     _oracle.lib().orc_fft_advance(of.ptr, dp(sre), dp(sim), st.delz, 1, 0.3, 0.4, 0.1)
     gre, gim, _ = f.download()
     assert relerr(gre + 1j * gim, of.re + 1j * of.im) < 1e-12
+
+
+@pytest.mark.parametrize("ngrid", [256, 128, 33])
+def test_fft_filter_mixed(gpu_ctx, ngrid):
+    """The filtered source in MIXED (FP64 work planes, FP32 field) against the
+    FP64 oracle: bar 1e-5 relative (FP32 round-off of one step is ~2e-7)."""
+    print(f"case fft filter mixed ngrid={ngrid}", flush=True)
+    c = cases.make_case(nslice=3, npart=1024, ngrid=ngrid, seed_power=1e6)
+    st = quad_step()
+    f = gpu_field(gpu_ctx, c, api.MIXED, kind=api.FFT)
+    f.init_solver(api.FFT, source_filter=True, xcut=0.3, ycut=0.4, sigmoid=0.1)
+    b = gpu_beam(gpu_ctx, c, api.MIXED)
+    of = OField(c.nslice, c.ngrid, c.gridmax, LAMBDA0, c.slicelength, 1, c.seed_re, c.seed_im)
+    f.track(b, st, st.delz)
+    o = c.beam
+    sre = np.zeros(c.nslice * c.ngrid ** 2)
+    sim = np.zeros_like(sre)
+    _oracle.lib().orc_deposit(c.nslice, ip(o.offsets), dp(o.x), dp(o.y), dp(o.gamma), dp(o.theta),
+                              of.ptr, st, dp(o.current), st.delz, dp(sre), dp(sim))
+    _oracle.lib().orc_fft_advance(of.ptr, dp(sre), dp(sim), st.delz, 1, 0.3, 0.4, 0.1)
+    gre, gim, _ = f.download()
+    err = relerr(gre + 1j * gim, of.re + 1j * of.im)
+    print(f"  field {err:.3e}", flush=True)
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("slips", [[1.0] * 5, [0.5, 0.5, 0.5, 2.5, -1.0, -1.0], [3.0]])
