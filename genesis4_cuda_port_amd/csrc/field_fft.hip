@@ -33,6 +33,34 @@ namespace {
 
 constexpr int kMaxEpt = 16;  // complex elements held per thread
 
+// v of the lane that DPP control CTRL selects (both 32-bit halves moved)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+// Sum of v over the 64 lanes of the wave without LDS: within each row of 16
+// lanes by DPP (lane xor 1, xor 2, then the mirrored lane of the 8 and of the
+// 16, each step pairing lanes whose partial sums cover disjoint halves),
+// then the four row sums read from lanes 0, 16, 32, 48.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
+}
+
 template <class P>
 struct C2;
 template <>
@@ -576,9 +604,7 @@ __global__ __launch_bounds__((RegCfg<P, G>::kThreads),
   if (partials) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      double a = acc[k];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      const double a = wave_total(acc[k]);
       if (lane == 0) red[wave][k] = a;
     }
     __syncthreads();
@@ -789,9 +815,7 @@ __global__ __launch_bounds__((ConvCfg<P, M>::kThreads)) void fft_cols_conv_kerne
     acc[2] = dx * dx * sw;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      double a = acc[k];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      const double a = wave_total(acc[k]);
       if (lane == 0) red[wave][k] = a;
     }
     __syncthreads();
